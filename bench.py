@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the MLS-MPM step on one MI355X.
+
+Workload (one GPU): 3D snow, four 64^3 cubes (1,048,576 particles) of side 0.2 at seeded
+centres with one seeded velocity per cube, 128^3 grid (dx = 1/128), FP32, dt = 1e-4.  A "step"
+is one MPMSimulation<3>::advance() (P2G -> grid update -> G2P) over all particles, with the
+particle state resident in HBM for the whole timed region.
+
+Prints one JSON line: particle-steps/s, the roofline of the dominant kernel (algorithmic bytes
+per launch / its average launch time measured with HIP events on the engine's stream) and the
+CPU baseline (the restated reference step, serial FP32, on a bounded sample of the same
+workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/sec fwd and fwd+bwd (2D/3D, FP32) vs plain GPU & host CPU path"
+
+# Algorithmic (minimum) HBM bytes per particle per launch, SoA FP32 (3D snow):
+#   P2G reads x, v, F, C, Jp, mass, volume = 3+3+9+9+3 floats = 108 B, writes 16 B per active node
+#   G2P reads x, F, Jp = 13 floats = 52 B, writes x, v, C, F, Jp = 25 floats = 100 B,
+#       reads 12 B (3 velocity floats) per active node
+#   grid update reads and writes 4 floats per active node = 32 B
+P2G_BYTES_PER_PARTICLE = {(3, 0): 108, (3, 1): 104, (3, 2): 104, (2, 0): 60, (2, 1): 56, (2, 2): 56}
+G2P_BYTES_PER_PARTICLE = {(3, 0): 152, (3, 1): 144, (3, 2): 144, (2, 0): 80, (2, 1): 72, (2, 2): 72}
+P2G_BYTES_PER_NODE = {2: 12, 3: 16}
+G2P_BYTES_PER_NODE = {2: 8, 3: 12}
+GRID_BYTES_PER_NODE = {2: 24, 3: 32}
+
+
+def load_traffic(kernel: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(HERE, "profiles", f"traffic_{kernel}_four_cubes_3d.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(sc, steps: int):
+    import numpy as np  # noqa: F401
+
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+
+    O.build()
+    p = sc.particles
+    st = O.State(x=p.x, v=p.v, F=p.F, C=p.C, Jp=p.Jp, mass=p.mass, volume=p.volume)
+    secs = O.time_steps(st, int(sc.material), sc.res, sc.dt, sc.E, sc.nu, sc.gravity, warmup=1, steps=steps)
+    return {
+        "value": p.n * steps / secs,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"same workload ({p.n} particles, res {sc.res}): 1 untimed + {steps} timed steps of the "
+                  f"restated serial reference step (oracle/, FP32, -O3 -ffast-math), {secs:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the engine has no multi-GPU path in this build "
+              "(scenes shard without communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    from nuclearmpm_amd import MPMSimulation, scenes
+
+    sc = scenes.four_cubes_3d()
+    p = sc.particles
+    n = p.n
+    dim, mat = sc.particles.dim, int(sc.material)
+
+    sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity)
+    sim.advance(args.warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(args.steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    total = t1 - t0
+    ms_per_step = 1e3 * total / args.steps
+    value = n * args.steps / total
+
+    # per-kernel device times with HIP events on the engine stream (a separate pass, so the events
+    # do not perturb the end-to-end number above)
+    ksteps = max(10, min(args.steps, 200))
+    sim.set_timing(True)
+    sim.reset_kernel_times()
+    sim.advance(ksteps)
+    sim.synchronize()
+    kt = {k: sim.kernel_time(k)[0] for k in ("p2g", "grid", "g2p")}
+    sim.set_timing(False)
+    g_active = sim.active_nodes()
+    finite = bool(__import__("numpy").isfinite(sim.particles().x).all())
+    sim.close()
+
+    p2g_bytes = P2G_BYTES_PER_PARTICLE[(dim, mat)] * n + P2G_BYTES_PER_NODE[dim] * g_active
+    g2p_bytes = G2P_BYTES_PER_PARTICLE[(dim, mat)] * n + G2P_BYTES_PER_NODE[dim] * g_active
+    grid_bytes = GRID_BYTES_PER_NODE[dim] * g_active
+    per_kernel = {
+        "p2g": {"avg_ms": kt["p2g"], "alg_bytes": p2g_bytes},
+        "grid": {"avg_ms": kt["grid"], "alg_bytes": grid_bytes},
+        "g2p": {"avg_ms": kt["g2p"], "alg_bytes": g2p_bytes},
+    }
+    for v in per_kernel.values():
+        v["achieved_GBs"] = v["alg_bytes"] / (v["avg_ms"] * 1e-3) / 1e9 if v["avg_ms"] > 0 else 0.0
+        v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
+    achieved = per_kernel[dom]["achieved_GBs"]
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {
+            "workload": "four_cubes_3d: 3D snow plain step, 4 cubes x 64^3 = 1,048,576 particles, side 0.2, "
+                        "res 128 (dx=1/128), dt 1e-4, E 1e4, nu 0.2, gravity -100",
+            "particles": n,
+            "res": sc.res,
+            "dim": dim,
+            "material": "snow",
+            "mode": "fwd",
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(dom),
+        },
+        "kernels": per_kernel,
+        "g_active": g_active,
+        "state_finite": finite,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

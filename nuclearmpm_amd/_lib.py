@@ -1,0 +1,124 @@
+"""ctypes binding of libnclr_mpm.so (the HIP engine's C ABI, include/nclr_mpm.h).
+
+The library is built in-tree by __graft_entry__.build() (or `make -C
+nuclearmpm_amd/csrc`).  Loading fails loudly when it is missing: there is no
+CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+try:  # load torch's HIP runtime first so the engine shares it (same SONAME libamdhip64.so.7)
+    import torch  # noqa: F401
+except Exception:  # pragma: no cover - torch is optional for the plain engine
+    torch = None
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnclr_mpm.so")
+
+MPM_OK = 0
+MPM_ERR_INVALID = 1
+MPM_ERR_OUT_OF_RANGE = 2
+MPM_ERR_HIP = 3
+MPM_ERR_UNSUPPORTED = 4
+
+
+class mpm_options(ctypes.Structure):
+    _fields_ = [
+        ("identity_mode", ctypes.c_int32),
+        ("sort_every", ctypes.c_int32),
+        ("p2g_variant", ctypes.c_int32),
+        ("block_threads", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("timing", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 10),
+    ]
+
+
+class EngineError(RuntimeError):
+    """Error reported by the HIP engine."""
+
+
+class EngineMissing(ImportError):
+    """libnclr_mpm.so is not built."""
+
+
+_lib = None
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EngineMissing(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C nuclearmpm_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    i32, i64, f32 = ctypes.c_int, ctypes.c_int64, ctypes.c_float
+    H = ctypes.c_void_p
+    L.mpm_default_options.argtypes = [ctypes.POINTER(mpm_options)]
+    L.mpm_default_options.restype = None
+    L.mpm_create.argtypes = [i32, i32, i32, f32, f32, f32, f32, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                             ctypes.POINTER(mpm_options), ctypes.POINTER(H)]
+    L.mpm_create.restype = i32
+    L.mpm_advance.argtypes = [H, i32]
+    L.mpm_advance.restype = i32
+    L.mpm_synchronize.argtypes = [H]
+    L.mpm_synchronize.restype = i32
+    L.mpm_get_particles.argtypes = [H, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mpm_get_particles.restype = i32
+    L.mpm_set_particles.argtypes = [H, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mpm_set_particles.restype = i32
+    L.mpm_get_grid.argtypes = [H, _vp, _vp]
+    L.mpm_get_grid.restype = i32
+    L.mpm_get_params.argtypes = [H, _c_float_p, _c_float_p, _c_float_p, _c_float_p]
+    L.mpm_get_params.restype = i32
+    L.mpm_num_particles.argtypes = [H]
+    L.mpm_num_particles.restype = i64
+    L.mpm_num_nodes.argtypes = [H]
+    L.mpm_num_nodes.restype = i64
+    L.mpm_steps_taken.argtypes = [H]
+    L.mpm_steps_taken.restype = i64
+    L.mpm_kernel_time.argtypes = [H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
+    L.mpm_kernel_time.restype = i32
+    L.mpm_reset_kernel_times.argtypes = [H]
+    L.mpm_reset_kernel_times.restype = i32
+    L.mpm_set_timing.argtypes = [H, i32]
+    L.mpm_set_timing.restype = i32
+    L.mpm_active_nodes.argtypes = [H, ctypes.POINTER(i64)]
+    L.mpm_active_nodes.restype = i32
+    L.mpm_destroy.argtypes = [H]
+    L.mpm_destroy.restype = i32
+    L.mpm_last_error.argtypes = []
+    L.mpm_last_error.restype = ctypes.c_char_p
+    L.mpm_version.argtypes = []
+    L.mpm_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    """Raise the Python counterpart of an MPM_ERR_* code (IndexError for out_of_range,
+    like the reference's std::out_of_range from std::vector::at)."""
+    if rc == MPM_OK:
+        return
+    msg = lib().mpm_last_error().decode()
+    if rc == MPM_ERR_OUT_OF_RANGE:
+        raise IndexError(msg)
+    if rc == MPM_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == MPM_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise EngineError(msg)
+
+
+def ptr(a):
+    """Address of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return ctypes.c_void_p(a.ctypes.data)

@@ -1,0 +1,263 @@
+"""Python mirror of the reference's engine API (src/nclr.h:20-87) over the HIP engine.
+
+    from nuclearmpm_amd import MPMSimulation, MaterialModel, Particle, cube
+    parts = [Particle(p, 0xED553B) for p in cube(2, 50, 0.4, 0.6)]
+    sim = MPMSimulation(parts, MaterialModel.kSnow)
+    sim.advance()
+    state = sim.particles()          # numpy arrays, original particle order
+
+Differences from the reference, all deliberate:
+  * particles() returns a Particles record of numpy arrays (x (n,d), v, F (n,d,d), C, Jp,
+    mass, volume, c) instead of a vector of structs;
+  * advance(steps) takes an optional step count (queued on the device without waiting);
+  * a stencil leaving the grid raises IndexError (the reference's std::out_of_range) at the next
+    call that reads results, and is raised for every axis, where the reference aliases silently
+    into the neighbouring row on the faster axes.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from enum import IntEnum
+from typing import Iterable, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, mpm_options, ptr
+
+
+class MaterialModel(IntEnum):
+    """src/nclr.h:57-61"""
+    kSnow = 0
+    kJelly = 1
+    kLiquid = 2
+
+
+def diag(dim: int, value: float, identity_mode: int = 0) -> np.ndarray:
+    """diag<dim>(value) of src/nclr_math.h:13-19: only (0,0) and (1,1) are set in reference mode."""
+    m = np.zeros((dim, dim), np.float32)
+    m[0, 0] = value
+    m[1, 1] = value
+    if identity_mode == 1:
+        np.fill_diagonal(m, value)
+    return m
+
+
+@dataclass
+class Particle:
+    """One particle as built by the reference ctor (src/nclr.h:46-47): F = diag<dim>(1), C = 0, Jp = 1."""
+    x: np.ndarray
+    c: int = 0
+    v: Optional[np.ndarray] = None
+    mass: float = 1.0
+    volume: float = 1.0
+
+
+@dataclass
+class Particles:
+    """Structure-of-arrays particle set (the reference's std::vector<Particle<dim>>)."""
+    x: np.ndarray
+    v: np.ndarray
+    F: np.ndarray
+    C: np.ndarray
+    Jp: np.ndarray
+    mass: np.ndarray
+    volume: np.ndarray
+    c: np.ndarray = field(default=None)
+
+    def __post_init__(self):
+        for k in ("x", "v", "F", "C", "Jp", "mass", "volume"):
+            setattr(self, k, np.ascontiguousarray(getattr(self, k), dtype=np.float32))
+        if self.c is None:
+            self.c = np.zeros(self.n, np.int32)
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def dim(self) -> int:
+        return int(self.x.shape[1])
+
+    def __len__(self) -> int:
+        return self.n
+
+    @staticmethod
+    def from_positions(x, v=None, mass=None, volume=None, c=None, identity_mode: int = 0) -> "Particles":
+        """Particles with the ctor defaults of src/nclr.h:46-47 at positions x (n, d)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, d = x.shape
+        F = np.broadcast_to(diag(d, 1.0, identity_mode), (n, d, d)).copy()
+        return Particles(
+            x=x,
+            v=np.zeros_like(x) if v is None else np.broadcast_to(np.asarray(v, np.float32), (n, d)).copy(),
+            F=F,
+            C=np.zeros((n, d, d), np.float32),
+            Jp=np.ones(n, np.float32),
+            mass=np.ones(n, np.float32) if mass is None else np.broadcast_to(np.asarray(mass, np.float32), (n,)).copy(),
+            volume=np.ones(n, np.float32) if volume is None else np.broadcast_to(np.asarray(volume, np.float32),
+                                                                                  (n,)).copy(),
+            c=None if c is None else np.broadcast_to(np.asarray(c, np.int32), (n,)).copy(),
+        )
+
+    @staticmethod
+    def from_list(parts: Sequence[Particle], identity_mode: int = 0) -> "Particles":
+        if len(parts) == 0:
+            raise ValueError("empty particle list: pass Particles with an explicit dimension")
+        x = np.stack([np.asarray(p.x, np.float32) for p in parts])
+        v = np.stack([np.zeros_like(x[0]) if p.v is None else np.asarray(p.v, np.float32) for p in parts])
+        return Particles.from_positions(x, v=v, mass=np.array([p.mass for p in parts], np.float32),
+                                        volume=np.array([p.volume for p in parts], np.float32),
+                                        c=np.array([p.c for p in parts], np.int32), identity_mode=identity_mode)
+
+    def copy(self) -> "Particles":
+        return Particles(*(getattr(self, k).copy() for k in ("x", "v", "F", "C", "Jp", "mass", "volume", "c")))
+
+
+def default_options(**overrides) -> mpm_options:
+    o = mpm_options()
+    lib().mpm_default_options(ctypes.byref(o))
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"unknown engine option {k!r}")
+        setattr(o, k, int(v))
+    return o
+
+
+class MPMSimulation:
+    """MPMSimulation<dim> (src/nclr.h:63-87) running on the MI355X engine."""
+
+    kBoundary = 3
+    kSnowHardening = np.float32(10.0)
+    kJellyHardening = np.float32(0.3)
+    kLiquidHardening = np.float32(1.0)
+
+    def __init__(self, particles: Union[Particles, Sequence[Particle]], model: MaterialModel, res: int = 64,
+                 dt: float = 1e-4, E: float = 1e4, nu: float = 0.2, gravity: float = -100.0, *,
+                 identity_mode: int = 0, sort_every: int = 0, p2g_variant: int = 0, timing: bool = False,
+                 device: int = 0):
+        if not isinstance(particles, Particles):
+            particles = Particles.from_list(list(particles), identity_mode=identity_mode)
+        self._dim = particles.dim
+        if self._dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        self._res = int(res)
+        self._model = MaterialModel(model)
+        self._colors = particles.c.copy()
+        self._n = particles.n
+        self.options = default_options(identity_mode=identity_mode, sort_every=sort_every,
+                                       p2g_variant=p2g_variant, timing=int(timing), device=device)
+        h = ctypes.c_void_p()
+        check(lib().mpm_create(self._dim, int(self._model), self._res, dt, E, nu, gravity, self._n,
+                               ptr(particles.x), ptr(particles.v), ptr(particles.F), ptr(particles.C),
+                               ptr(particles.Jp), ptr(particles.mass), ptr(particles.volume),
+                               ctypes.byref(self.options), ctypes.byref(h)))
+        self._h = h
+        mu = ctypes.c_float()
+        lam = ctypes.c_float()
+        dx = ctypes.c_float()
+        inv_dx = ctypes.c_float()
+        check(lib().mpm_get_params(self._h, ctypes.byref(mu), ctypes.byref(lam), ctypes.byref(dx),
+                                   ctypes.byref(inv_dx)))
+        self.mu_0 = np.float32(mu.value)
+        self.lambda_0 = np.float32(lam.value)
+        self.dx = np.float32(dx.value)
+        self.inv_dx = np.float32(inv_dx.value)
+        self._cache = None
+
+    # -- reference API ---------------------------------------------------------------------
+    def advance(self, steps: int = 1) -> None:
+        """advance() (src/nclr.h:80-84), `steps` times; queued without waiting."""
+        self._cache = None
+        check(lib().mpm_advance(self._handle(), int(steps)))
+
+    def particles(self) -> Particles:
+        """particles() (src/nclr.h:86): the current state, original particle order."""
+        if self._cache is None:
+            n, d = self._n, self._dim
+            out = Particles(x=np.empty((n, d), np.float32), v=np.empty((n, d), np.float32),
+                            F=np.empty((n, d, d), np.float32), C=np.empty((n, d, d), np.float32),
+                            Jp=np.empty(n, np.float32), mass=np.empty(n, np.float32),
+                            volume=np.empty(n, np.float32), c=self._colors.copy())
+            check(lib().mpm_get_particles(self._handle(), ptr(out.x), ptr(out.v), ptr(out.F), ptr(out.C),
+                                          ptr(out.Jp), ptr(out.mass), ptr(out.volume)))
+            self._cache = out
+        return self._cache
+
+    def grid(self):
+        """grid() (src/nclr.h:87): (velocity (res+1,)*d + (d,), mass (res+1,)*d) after the last grid update."""
+        nodes = (self._res + 1) ** self._dim
+        vel = np.empty((nodes, self._dim), np.float32)
+        mass = np.empty(nodes, np.float32)
+        check(lib().mpm_get_grid(self._handle(), ptr(vel), ptr(mass)))
+        shape = (self._res + 1,) * self._dim
+        return vel.reshape(shape + (self._dim,)), mass.reshape(shape)
+
+    # -- extras ------------------------------------------------------------------------------
+    def set_particles(self, p: Particles) -> None:
+        """Overwrite the particle state (same count and dimension)."""
+        if p.n != self._n or p.dim != self._dim:
+            raise ValueError("particle count/dimension mismatch")
+        self._cache = None
+        check(lib().mpm_set_particles(self._handle(), ptr(p.x), ptr(p.v), ptr(p.F), ptr(p.C), ptr(p.Jp),
+                                      ptr(p.mass), ptr(p.volume)))
+
+    def synchronize(self) -> None:
+        check(lib().mpm_synchronize(self._handle()))
+
+    def kernel_time(self, name: str):
+        """(average ms per launch, launches) of kernel `name` ("sort", "p2g", "grid", "g2p"); needs timing=True."""
+        avg = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        check(lib().mpm_kernel_time(self._handle(), name.encode(), ctypes.byref(avg), ctypes.byref(cnt)))
+        return avg.value, cnt.value
+
+    def set_timing(self, enable: bool) -> None:
+        check(lib().mpm_set_timing(self._handle(), int(bool(enable))))
+
+    def reset_kernel_times(self) -> None:
+        check(lib().mpm_reset_kernel_times(self._handle()))
+
+    def active_nodes(self) -> int:
+        c = ctypes.c_int64()
+        check(lib().mpm_active_nodes(self._handle(), ctypes.byref(c)))
+        return c.value
+
+    @property
+    def steps_taken(self) -> int:
+        return int(lib().mpm_steps_taken(self._handle()))
+
+    @property
+    def dim(self) -> int:
+        return self._dim
+
+    @property
+    def res(self) -> int:
+        return self._res
+
+    @property
+    def n(self) -> int:
+        return self._n
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mpm_destroy(self._h)
+            self._h = None
+
+    def _handle(self):
+        if not self._h:
+            raise ValueError("simulation is closed")
+        return self._h
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
