@@ -111,6 +111,11 @@ int mpm_set_timing(mpm_handle h, int enable);
  * roofline bytes). */
 int mpm_active_nodes(mpm_handle h, int64_t* count);
 
+/* The engine's stable counting sort (the per-step particle-by-tile sort) on n host keys:
+ * perm[j] = index of the j-th key in stable order by the low `bits` bits (no reference
+ * counterpart; the reference steps particles in index order, src/nclr.h:112,169). */
+int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int device);
+
 /* Releases the device memory and stream of h. */
 int mpm_destroy(mpm_handle h);
 

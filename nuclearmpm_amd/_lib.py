@@ -90,6 +90,8 @@ def lib() -> ctypes.CDLL:
     L.mpm_set_timing.restype = i32
     L.mpm_active_nodes.argtypes = [H, ctypes.POINTER(i64)]
     L.mpm_active_nodes.restype = i32
+    L.mpm_sort_keys.argtypes = [_vp, i64, i32, _vp, i32]
+    L.mpm_sort_keys.restype = i32
     L.mpm_destroy.argtypes = [H]
     L.mpm_destroy.restype = i32
     L.mpm_last_error.argtypes = []
@@ -113,6 +115,17 @@ def check(rc: int) -> None:
     if rc == MPM_ERR_UNSUPPORTED:
         raise NotImplementedError(msg)
     raise EngineError(msg)
+
+
+def sort_keys(keys, bits: int, device: int = 0):
+    """The engine's stable GPU counting sort: permutation ordering `keys` (uint32) by their low
+    `bits` bits, ties in input order."""
+    import numpy as np
+
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    perm = np.empty(k.shape[0], np.uint32)
+    check(lib().mpm_sort_keys(ptr(k), k.shape[0], int(bits), ptr(perm), int(device)))
+    return perm
 
 
 def ptr(a):

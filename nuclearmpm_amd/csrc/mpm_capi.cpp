@@ -1,9 +1,10 @@
 // C ABI of the MLS-MPM engine (declarations and contracts: include/nclr_mpm.h).
 //
-// One handle = one simulation = one HIP stream.  The handle owns the
-// structure-of-arrays particle state, the blocked grid buffers and a device
-// status word that kernels set when a particle leaves the grid; the status is
-// read back (and the error reported) only by synchronising calls, so that
+// One handle = one simulation = one HIP stream.  The handle owns two ping-pong buffers of
+// structure-of-arrays particle state (G2P writes the state in tile order into the other buffer),
+// the original particle index of every slot, the sort buffers, the per-tile P2G slots, the
+// tile-blocked updated grid and a device status word that kernels set when a particle leaves
+// the grid.  The status is read back (and the error reported) only by synchronising calls, so
 // mpm_advance() never waits on the device.
 #include <hip/hip_runtime.h>
 
@@ -24,9 +25,9 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                             \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
+#define HIP_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
         if (e_ != hipSuccess) return fail(MPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
@@ -43,25 +44,30 @@ struct KernelTimer {
     int64_t launches = 0;
 };
 
-constexpr int kBlockNodes = 64;
-
-int block_log(int dim) { return dim == 2 ? 3 : 2; }
+template <typename T>
+hipError_t dmalloc(T** p, size_t count) {
+    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
 
 }  // namespace
 
 struct nclr_mpm_sim {
     int dim = 0, material = 0, res = 0;
-    float E = 0, nu = 0;
     mpm_options opts{};
     nclr_dev::StepParams p{};
     hipStream_t stream = nullptr;
-    float* state = nullptr;
-    float4* acc = nullptr;
+    float* state[2] = {nullptr, nullptr};
+    uint32_t* pid[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint32_t *keys = nullptr, *perm = nullptr, *keys_tmp = nullptr, *perm_tmp = nullptr, *hist = nullptr;
+    uint32_t *tile_start = nullptr, *tile_end = nullptr, *item_offset = nullptr;
+    float4* slots = nullptr;
     float4* gv = nullptr;
     uint32_t* stamps = nullptr;
     uint32_t* status = nullptr;
-    uint32_t* active_count = nullptr;
-    int64_t nblocks = 0;
+    uint32_t* active = nullptr;
+    unsigned long long* active_sum = nullptr;
+    int64_t node_tiles = 0;
     int64_t steps = 0;
     std::vector<PendingEvent> pending;
     std::vector<hipEvent_t> pool;
@@ -72,13 +78,23 @@ struct nclr_mpm_sim {
         a.dim = dim;
         a.material = material;
         a.p = p;
-        a.state = state;
-        a.acc = acc;
+        a.state_in = state[cur];
+        a.state_out = state[cur ^ 1];
+        a.pid_in = pid[cur];
+        a.pid_out = pid[cur ^ 1];
+        a.keys = keys;
+        a.perm = perm;
+        a.keys_tmp = keys_tmp;
+        a.perm_tmp = perm_tmp;
+        a.hist = hist;
+        a.tile_start = tile_start;
+        a.tile_end = tile_end;
+        a.item_offset = item_offset;
+        a.slots = slots;
         a.gv = gv;
         a.stamps = stamps;
-        a.nblocks = nblocks;
         a.status = status;
-        a.active_count = active_count;
+        a.active = active;
         a.stream = stream;
         return a;
     }
@@ -139,12 +155,15 @@ struct nclr_mpm_sim {
         p.stamp = p.stamp + 1;
         if (p.stamp == 0) p.stamp = 1;
         const nclr_host::LaunchArgs a = args();
-        int rc = timed(kP2G, [&] { return nclr_host::launch_p2g(a); });
+        int rc = timed(kSort, [&] { return nclr_host::launch_sort(a); });
+        if (rc != MPM_OK) return rc;
+        rc = timed(kP2G, [&] { return nclr_host::launch_p2g(a); });
         if (rc != MPM_OK) return rc;
         rc = timed(kGrid, [&] { return nclr_host::launch_grid(a); });
         if (rc != MPM_OK) return rc;
         rc = timed(kG2P, [&] { return nclr_host::launch_g2p(a); });
         if (rc != MPM_OK) return rc;
+        if (p.n > 0) cur ^= 1;
         steps += 1;
         return MPM_OK;
     }
@@ -170,7 +189,18 @@ struct nclr_mpm_sim {
         return MPM_OK;
     }
 
-    // Host particle-major arrays -> device SoA for the fields given (NULL = keep `fallback`).
+    // Device state -> host SoA image (physical slot order) and the slot -> particle map.
+    int download(std::vector<float>& host, std::vector<uint32_t>& ids) {
+        host.assign(size_t(nf()) * size_t(p.stride), 0.0f);
+        ids.assign(size_t(p.stride), 0u);
+        HIP_TRY(hipMemcpyAsync(host.data(), state[cur], host.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(ids.data(), pid[cur], ids.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return MPM_OK;
+    }
+
+    // Host particle-major arrays (NULL = keep) -> host SoA image in original particle order ->
+    // device; resets the slot order to the particle order.
     int upload(const float* x, const float* v, const float* F, const float* C, const float* Jp, const float* mass,
                const float* volume, std::vector<float>& host) {
         const int D = dim;
@@ -185,36 +215,41 @@ struct nclr_mpm_sim {
                 if (F) host[(fF + k) * stride + i] = F[i * D * D + k];
                 if (C) host[(fC + k) * stride + i] = C[i * D * D + k];
             }
-            if (Jp) host[(fJ)*stride + i] = Jp[i];
+            if (Jp) host[fJ * stride + i] = Jp[i];
             if (mass) host[(fJ + 1) * stride + i] = mass[i];
             if (volume) host[(fJ + 2) * stride + i] = volume[i];
         }
-        HIP_TRY(hipMemcpyAsync(state, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        cur = 0;
+        HIP_TRY(hipMemcpyAsync(state[0], host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_TRY(nclr_host::launch_iota_u32(pid[0], p.stride, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return MPM_OK;
     }
 
-    int download(std::vector<float>& host) {
-        host.assign(size_t(nf()) * size_t(p.stride), 0.0f);
-        HIP_TRY(hipMemcpyAsync(host.data(), state, host.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
-        return MPM_OK;
+    // Host SoA image in slot order -> the same image in original particle order.
+    void unpermute(std::vector<float>& host, const std::vector<uint32_t>& ids) const {
+        std::vector<float> out(host.size(), 0.0f);
+        const int64_t n = p.n, stride = p.stride;
+        for (int f = 0; f < nf(); ++f)
+            for (int64_t s = 0; s < n; ++s) out[f * stride + ids[s]] = host[f * stride + s];
+        host.swap(out);
     }
 
     void release() {
         if (opts.timing) (void)resolve_events();
         for (auto e : pool) (void)hipEventDestroy(e);
         pool.clear();
-        if (state) (void)hipFree(state);
-        if (acc) (void)hipFree(acc);
-        if (gv) (void)hipFree(gv);
-        if (stamps) (void)hipFree(stamps);
-        if (status) (void)hipFree(status);
-        if (active_count) (void)hipFree(active_count);
+        void* bufs[] = {state[0], state[1], pid[0], pid[1], keys, perm, keys_tmp, perm_tmp, hist, tile_start,
+                        tile_end, item_offset, slots, gv, stamps, status, active, active_sum};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        state[0] = state[1] = nullptr;
+        pid[0] = pid[1] = nullptr;
+        keys = perm = keys_tmp = perm_tmp = hist = tile_start = tile_end = item_offset = nullptr;
+        slots = gv = nullptr;
+        stamps = status = active = nullptr;
+        active_sum = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
-        state = nullptr;
-        acc = gv = nullptr;
-        stamps = status = active_count = nullptr;
         stream = nullptr;
     }
 };
@@ -224,12 +259,6 @@ extern "C" {
 void mpm_default_options(mpm_options* opts) {
     if (!opts) return;
     std::memset(opts, 0, sizeof(*opts));
-    opts->identity_mode = 0;
-    opts->sort_every = 0;
-    opts->p2g_variant = 0;
-    opts->block_threads = 0;
-    opts->device = 0;
-    opts->timing = 0;
 }
 
 int mpm_create(int dim, int material, int res, float dt, float E, float nu, float gravity, int64_t n, const float* x,
@@ -239,7 +268,7 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
     *out = nullptr;
     if (dim != 2 && dim != 3) return fail(MPM_ERR_INVALID, "dim must be 2 or 3");
     if (material < MPM_SNOW || material > MPM_LIQUID) return fail(MPM_ERR_INVALID, "unknown material model");
-    if (res < 3 || res > 4096) return fail(MPM_ERR_INVALID, "res must be in [3, 4096]");
+    if (res < 3 || res > (dim == 3 ? 512 : 8192)) return fail(MPM_ERR_INVALID, "res out of range");
     if (n < 0 || n > (int64_t(1) << 31) - 64) return fail(MPM_ERR_INVALID, "particle count out of range");
     if (n > 0 && !x) return fail(MPM_ERR_INVALID, "x must not be NULL");
     if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(MPM_ERR_INVALID, "dt must be positive and finite");
@@ -252,12 +281,18 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
     h->dim = dim;
     h->material = material;
     h->res = res;
-    h->E = E;
-    h->nu = nu;
     auto& p = h->p;
     p.res = res;
-    const int B = 1 << block_log(dim);
-    p.nb = (res + 1 + B - 1) / B;
+    const int lg = nclr_host::tile_log(dim);
+    p.nt = ((res - 2) >> lg) + 1;  // stencil bases lie in [0, res-2]
+    p.ntn = (res >> lg) + 1;       // nodes lie in [0, res]
+    uint64_t ntiles = 1;
+    h->node_tiles = 1;
+    for (int a = 0; a < dim; ++a) {
+        ntiles *= uint64_t(p.nt);
+        h->node_tiles *= p.ntn;
+    }
+    p.ntiles = uint32_t(ntiles);
     p.identity_mode = h->opts.identity_mode;
     p.stamp = 0;
     p.dt = dt;
@@ -270,8 +305,6 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
     p.allowed = float(double(p.dx) * 0.9 / double(dt));
     p.n = n;
     p.stride = std::max<int64_t>(64, (n + 63) / 64 * 64);
-    h->nblocks = 1;
-    for (int a = 0; a < dim; ++a) h->nblocks *= p.nb;
 
     auto cleanup = [&](int rc) {
         h->release();
@@ -281,20 +314,28 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
     if (hipSetDevice(h->opts.device) != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "hipSetDevice failed"));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipStreamCreate failed"));
-    const size_t nodes = size_t(h->nblocks) * kBlockNodes;
     const size_t state_floats = size_t(h->nf()) * size_t(p.stride);
-    if (hipMalloc(&h->state, state_floats * sizeof(float)) != hipSuccess ||
-        hipMalloc(&h->acc, nodes * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&h->gv, nodes * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&h->stamps, size_t(h->nblocks) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&h->status, sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&h->active_count, 2 * sizeof(uint32_t)) != hipSuccess)
+    const size_t nbins = size_t(ntiles) + 1;
+    const size_t gv_nodes = size_t(h->node_tiles) * nclr_host::tile_nodes(dim);
+    const size_t slot_nodes = size_t(nclr_host::max_items(dim, n, uint32_t(ntiles))) * nclr_host::region_nodes(dim);
+    const size_t nn = size_t(std::max<int64_t>(n, 1));
+    if (dmalloc(&h->state[0], state_floats) != hipSuccess || dmalloc(&h->state[1], state_floats) != hipSuccess ||
+        dmalloc(&h->pid[0], size_t(p.stride)) != hipSuccess || dmalloc(&h->pid[1], size_t(p.stride)) != hipSuccess ||
+        dmalloc(&h->keys, nn) != hipSuccess || dmalloc(&h->perm, nn) != hipSuccess ||
+        dmalloc(&h->keys_tmp, nn) != hipSuccess || dmalloc(&h->perm_tmp, nn) != hipSuccess ||
+        dmalloc(&h->hist, size_t(256) * size_t(nclr_host::sort_tiles(int64_t(nn))) + 256) != hipSuccess ||
+        dmalloc(&h->tile_start, nbins) != hipSuccess || dmalloc(&h->tile_end, nbins) != hipSuccess ||
+        dmalloc(&h->item_offset, nbins) != hipSuccess || dmalloc(&h->slots, slot_nodes) != hipSuccess ||
+        dmalloc(&h->gv, gv_nodes) != hipSuccess || dmalloc(&h->stamps, size_t(h->node_tiles)) != hipSuccess ||
+        dmalloc(&h->status, 1) != hipSuccess || dmalloc(&h->active, size_t(h->node_tiles)) != hipSuccess ||
+        dmalloc(&h->active_sum, 1) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
-    if (nclr_host::launch_zero_f4(h->acc, int64_t(nodes), h->stream) != hipSuccess ||
-        nclr_host::launch_zero_f4(h->gv, int64_t(nodes), h->stream) != hipSuccess ||
-        nclr_host::launch_fill_u32(h->stamps, h->nblocks, 0u, h->stream) != hipSuccess ||
+    if (nclr_host::launch_zero_f4(h->gv, int64_t(gv_nodes), h->stream) != hipSuccess ||
+        nclr_host::launch_zero_f4(h->slots, int64_t(slot_nodes), h->stream) != hipSuccess ||
+        nclr_host::launch_fill_u32(h->stamps, h->node_tiles, 0u, h->stream) != hipSuccess ||
         nclr_host::launch_fill_u32(h->status, 1, 0u, h->stream) != hipSuccess ||
-        nclr_host::launch_fill_u32(h->active_count, 2, 0u, h->stream) != hipSuccess)
+        nclr_host::launch_fill_u32(h->active, h->node_tiles, 0u, h->stream) != hipSuccess ||
+        nclr_host::launch_fill_u32(h->pid[1], p.stride, 0u, h->stream) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "device initialisation failed"));
 
     // Defaults of the Particle constructor (src/nclr.h:46-47): v = 0, F = diag<dim>(1), C = 0,
@@ -311,6 +352,10 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
     }
     const int rc = h->upload(x, v, F, C, Jp, mass, volume, host);
     if (rc != MPM_OK) return cleanup(rc);
+    if (hipMemcpyAsync(h->state[1], h->state[0], state_floats * sizeof(float), hipMemcpyDeviceToDevice, h->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return cleanup(fail(MPM_ERR_HIP, "device initialisation failed"));
     *out = h;
     return MPM_OK;
 }
@@ -337,8 +382,10 @@ int mpm_get_particles(mpm_handle h, float* x, float* v, float* F, float* C, floa
     int rc = h->sync();
     if (rc != MPM_OK) return rc;
     std::vector<float> host;
-    rc = h->download(host);
+    std::vector<uint32_t> ids;
+    rc = h->download(host, ids);
     if (rc != MPM_OK) return rc;
+    h->unpermute(host, ids);
     const int D = h->dim;
     const int64_t n = h->p.n, stride = h->p.stride;
     const int fX = 0, fV = D, fF = 2 * D, fC = 2 * D + D * D, fJ = 2 * D + 2 * D * D;
@@ -364,8 +411,10 @@ int mpm_set_particles(mpm_handle h, const float* x, const float* v, const float*
     int rc = h->sync();
     if (rc != MPM_OK) return rc;
     std::vector<float> host;
-    rc = h->download(host);
+    std::vector<uint32_t> ids;
+    rc = h->download(host, ids);
     if (rc != MPM_OK) return rc;
+    h->unpermute(host, ids);
     return h->upload(x, v, F, C, Jp, mass, volume, host);
 }
 
@@ -376,8 +425,8 @@ int mpm_get_grid(mpm_handle h, float* velocity, float* mass) {
     const int64_t total = h->num_nodes();
     float* dvel = nullptr;
     float* dmass = nullptr;
-    HIP_TRY(hipMalloc(&dvel, size_t(total) * h->dim * sizeof(float)));
-    if (hipMalloc(&dmass, size_t(total) * sizeof(float)) != hipSuccess) {
+    HIP_TRY(dmalloc(&dvel, size_t(total) * h->dim));
+    if (dmalloc(&dmass, size_t(total)) != hipSuccess) {
         (void)hipFree(dvel);
         return fail(MPM_ERR_HIP, "hipMalloc failed");
     }
@@ -443,9 +492,41 @@ int mpm_active_nodes(mpm_handle h, int64_t* count) {
     if (!h || !count) return fail(MPM_ERR_INVALID, "NULL argument");
     int rc = h->sync();
     if (rc != MPM_OK) return rc;
-    uint32_t c[2] = {0, 0};
-    HIP_TRY(hipMemcpy(c, h->active_count, sizeof(c), hipMemcpyDeviceToHost));
-    *count = h->p.stamp == 0 ? 0 : int64_t(c[h->p.stamp & 1]);
+    if (h->p.stamp == 0) {
+        *count = 0;
+        return MPM_OK;
+    }
+    unsigned long long c = 0;
+    HIP_TRY(nclr_host::launch_active_sum(h->args(), h->active_sum));
+    HIP_TRY(hipMemcpyAsync(&c, h->active_sum, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *count = int64_t(c);
+    return MPM_OK;
+}
+
+int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int device) {
+    if (n < 0 || (n > 0 && (!keys || !perm))) return fail(MPM_ERR_INVALID, "bad arguments");
+    if (bits < 0 || bits > 32) return fail(MPM_ERR_INVALID, "bits must be in [0, 32]");
+    if (n == 0) return MPM_OK;
+    if (n > (int64_t(1) << 31)) return fail(MPM_ERR_INVALID, "too many keys");
+    HIP_TRY(hipSetDevice(device));
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    uint32_t *dk = nullptr, *dv = nullptr, *tk = nullptr, *tv = nullptr, *hist = nullptr;
+    hipError_t e = dmalloc(&dk, size_t(n));
+    if (e == hipSuccess) e = dmalloc(&dv, size_t(n));
+    if (e == hipSuccess) e = dmalloc(&tk, size_t(n));
+    if (e == hipSuccess) e = dmalloc(&tv, size_t(n));
+    if (e == hipSuccess) e = dmalloc(&hist, size_t(256) * size_t(nclr_host::sort_tiles(n)) + 256);
+    if (e == hipSuccess) e = hipMemcpyAsync(dk, keys, size_t(n) * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = nclr_host::radix_sort_pairs(dk, dv, tk, tv, hist, n, bits, true, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(perm, dv, size_t(n) * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    void* bufs[] = {dk, dv, tk, tv, hist};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("mpm_sort_keys: ") + hipGetErrorString(e));
     return MPM_OK;
 }
 
@@ -461,7 +542,8 @@ int mpm_destroy(mpm_handle h) {
 const char* mpm_last_error(void) { return g_err.c_str(); }
 
 const char* mpm_version(void) {
-    return "nclr_mpm 0.1 gfx950 (fp32, no fp contraction, IEEE div/sqrt, SoA particles, blocked grid)";
+    return "nclr_mpm 0.2 gfx950 (fp32, no fp contraction, IEEE div/sqrt, tile-sorted SoA particles, "
+           "LDS tile P2G)";
 }
 
 }  // extern "C"

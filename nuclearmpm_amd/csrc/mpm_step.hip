@@ -1,0 +1,819 @@
+// HIP kernels of the MLS-MPM step for gfx950 (MI355X).
+//
+// Reference step: MPMSimulation<dim>::advance() = p2g(); grid_op(); g2p(); (src/nclr.h:80-84).
+//
+// Device data layout
+//   particles   structure-of-arrays, field f of slot i at st[f*stride + i] (Layout<D>), in two
+//               ping-pong buffers; pid[i] is the original particle index of slot i.
+//   tiles       particles are keyed by the tile of their stencil base cell (S^D cells, S = 16 in
+//               2D, 8 in 3D) and counting-sorted every step (mpm_sort.hip), giving perm:
+//               sorted position -> slot, and [tile_start, tile_end) per tile.
+//   slots       per particle tile, its P2G region of (S+2)^D nodes as float4 {mom, m}.
+//   gv          the updated grid, node tiles of S^D nodes, float4 {v, m}; stamps[node tile] ==
+//               current stamp marks the tiles this step touched.
+//
+// Kernels per step
+//   keys        tile key per particle (out-of-grid particles get key ntiles and set the status)
+//   sort        stable radix counting sort of (key, slot), tile bounds, P2G work items
+//   p2g         one workgroup per (tile, chunk of <= CHUNK particles): LDS scatter of the
+//               3^D stencil, region written to the tile's slot (plain stores; atomics when the
+//               tile has several chunks)
+//   grid        one workgroup per node tile: sums the <= 2^D slots covering each node and
+//               applies grid_op (src/nclr.h:263-310)
+//   g2p         per sorted position j: gathers particle perm[j], writes it to slot j of the
+//               other buffer, so the state stays physically sorted by tile
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mpm_internal.hpp"
+#include "mpm_math.hpp"
+
+namespace nclr_dev {
+
+template <int D>
+struct Layout {
+    static constexpr int X = 0;
+    static constexpr int V = D;
+    static constexpr int F = 2 * D;
+    static constexpr int C = 2 * D + D * D;
+    static constexpr int JP = 2 * D + 2 * D * D;
+    static constexpr int MASS = JP + 1;
+    static constexpr int VOL = JP + 2;
+    static constexpr int NF = JP + 3;
+};
+
+// base = trunc(x*inv_dx - 0.5), fx = x*inv_dx - base (src/nclr.h:115,121), with the stencil
+// nodes base..base+2 required to lie in [0, res] on every axis.
+template <int D>
+NCLR_DI bool stencil(const float (&x)[D], const StepParams& p, int (&base)[D], float (&fx)[D], float (&w)[D][3]) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const float s = x[a] * p.inv_dx;
+        const float b = s - 0.5f;
+        ok = ok && (b > -1.0f) && (b < float(p.res - 1));
+        base[a] = ok ? int(b) : 0;
+        fx[a] = s - float(base[a]);
+        bspline_weights(fx[a], w[a]);
+    }
+    return ok;
+}
+
+NCLR_DI float ldf(const float* __restrict__ st, int64_t stride, int f, int64_t i) { return st[int64_t(f) * stride + i]; }
+
+template <int MAT>
+NCLR_DI float hardening(float Jp) {
+    if constexpr (MAT == MPM_SNOW) {
+        return float(exp(10.0 * (1.0 - double(Jp))));
+    } else if constexpr (MAT == MPM_JELLY) {
+        return 0.3f;
+    } else {
+        return 1.0f;
+    }
+}
+
+// first_piola_kirchoff_stress (src/nclr.h:313-337): -(dt V0)(4 inv_dx^2 PF) + m C with
+// PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry.
+template <int D, int MAT>
+NCLR_DI M<D> fused_affine(const M<D>& F, const M<D>& C, float Jp, float mass, float vol, const StepParams& p) {
+    const float e = hardening<MAT>(Jp);
+    const float mu = p.mu0 * e;
+    const float lambda = p.lambda0 * e;
+    const float J = det(F);
+    const M<D> R = polar_rotation(F);
+    const float Dinv = 4.0f * p.inv_dx * p.inv_dx;
+    const float two_mu = 2.0f * mu;
+    M<D> FmR;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) FmR.a[a][b] = two_mu * (F.a[a][b] - R.a[a][b]);
+    const M<D> PF = matmul_bt<D>(FmR, F);
+    const float lam_term = lambda * (J - 1.0f) * J;
+    const float neg_dtv = -(p.dt * vol);
+    M<D> A;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) A.a[a][b] = neg_dtv * (Dinv * (PF.a[a][b] + lam_term)) + mass * C.a[a][b];
+    return A;
+}
+
+template <int D>
+NCLR_DI void offsets(int o, int (&off)[D]) {
+    if constexpr (D == 2) {
+        off[0] = o / 3;
+        off[1] = o % 3;
+    } else {
+        off[0] = o / 9;
+        off[1] = (o / 3) % 3;
+        off[2] = o % 3;
+    }
+}
+
+// Index of node n in the tile-blocked updated grid.
+template <int D>
+NCLR_DI int64_t gv_index(const int (&n)[D], int ntn) {
+    using TL = Tile<D>;
+    int64_t t = 0;
+    int local = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        t = t * ntn + (n[a] >> TL::LOG);
+        local = local * TL::S + (n[a] & (TL::S - 1));
+    }
+    return t * TL::NODES + local;
+}
+
+template <int D>
+NCLR_DI uint32_t tile_key(const int (&base)[D], int nt) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) t = t * uint32_t(nt) + uint32_t(base[a] >> Tile<D>::LOG);
+    return t;
+}
+
+template <int D>
+NCLR_DI void tile_coords(uint32_t t, int n, int (&c)[D]) {
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) {
+        c[a] = int(t % uint32_t(n));
+        t /= uint32_t(n);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// keys + tile bounds + work items
+// ---------------------------------------------------------------------------------------------
+// Tile key of every particle; also clears the tile bounds that k_tile_bounds fills after the sort.
+template <int D>
+__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ st, StepParams p, uint32_t* __restrict__ keys,
+                                              uint32_t* __restrict__ status, uint32_t* __restrict__ tile_start,
+                                              uint32_t* __restrict__ tile_end) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i <= int64_t(p.ntiles)) {
+        tile_start[i] = 0;
+        tile_end[i] = 0;
+    }
+    if (i >= p.n) return;
+    float x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = ldf(st, p.stride, Layout<D>::X + a, i);
+    int base[D];
+    float fx[D], w[D][3];
+    if (stencil<D>(x, p, base, fx, w)) {
+        keys[i] = tile_key<D>(base, p.nt);
+    } else {
+        keys[i] = p.ntiles;
+        atomicOr(status, kStatusOutOfRange);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict__ keys, int64_t n,
+                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t k = keys[j];
+    if (j == 0 || keys[j - 1] != k) start[k] = uint32_t(j);
+    if (j == n - 1 || keys[j + 1] != k) end[k] = uint32_t(j + 1);
+}
+
+// P2G work items: ceil(count / CHUNK) per tile, exclusive scan into item_offset[0..ntiles]
+// (item_offset[ntiles] = total).  One workgroup of 1024 threads, coalesced rounds of 4096 tiles.
+template <int D>
+__global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                                uint32_t ntiles, uint32_t* __restrict__ item_offset) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b <= ntiles; b += 4096) {
+        uint32_t v[4];
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t tile = b + uint32_t(t) * 4 + k;
+            v[k] = tile < ntiles ? (end[tile] - start[tile] + Tile<D>::CHUNK - 1) / Tile<D>::CHUNK : 0u;
+            s += v[k];
+        }
+        part[t] = s;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t u = t >= off ? part[t - off] : 0u;
+            __syncthreads();
+            part[t] += u;
+            __syncthreads();
+        }
+        uint32_t run = carry + part[t] - s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t tile = b + uint32_t(t) * 4 + k;
+            if (tile <= ntiles) item_offset[tile] = run;
+            run += v[k];
+        }
+        carry += part[1023];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// P2G (src/nclr.h:104-165): LDS tile scatter, region -> slot
+// ---------------------------------------------------------------------------------------------
+template <int D, int MAT>
+__global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, const uint32_t* __restrict__ perm,
+                                                  const uint32_t* __restrict__ tile_start,
+                                                  const uint32_t* __restrict__ tile_end,
+                                                  const uint32_t* __restrict__ item_offset, StepParams p,
+                                                  float4* __restrict__ slots) {
+    using TL = Tile<D>;
+    using L = Layout<D>;
+    constexpr int REG = TL::REGION;
+    constexpr int PPT = TL::CHUNK / 256;  // particles per thread
+    // LDS accumulators: exact fixed-point sums (64-bit integer atomics are an order of magnitude
+    // faster than float atomics in LDS on gfx950, and make the sum independent of lane order)
+    __shared__ unsigned long long acc[4 * REG];
+    __shared__ uint32_t nonfinite[REG];  // bit c: a non-finite contribution hit channel c
+    __shared__ float s_bound[4][4];      // per wave, per channel
+    __shared__ uint32_t s_tile, s_chunk;
+    const uint32_t item = blockIdx.x;
+    if (item >= item_offset[p.ntiles]) return;
+    if (threadIdx.x == 0) {
+        uint32_t lo = 0, hi = p.ntiles;  // item_offset[lo] <= item < item_offset[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (item_offset[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        s_tile = lo;
+        s_chunk = item - item_offset[lo];
+    }
+    for (int r = threadIdx.x; r < 4 * REG; r += blockDim.x) acc[r] = 0ull;
+    for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    int torg[D];
+    tile_coords<D>(tile, p.nt, torg);
+#pragma unroll
+    for (int a = 0; a < D; ++a) torg[a] *= TL::S;
+    const uint32_t j0 = tile_start[tile] + s_chunk * TL::CHUNK;
+    const uint32_t j1 = min(j0 + uint32_t(TL::CHUNK), tile_end[tile]);
+
+    // pass 1: per particle the fused affine matrix, kept in registers, and a bound of the sum of
+    // |contributions| per channel: sum_nodes |w| (|m v_c| + 2.5 dx sum_k |A_ck|)
+    bool valid[PPT];
+    int rb[PPT];
+    float fxs[PPT][D], mvs[PPT][D], ms[PPT];
+    M<D> As[PPT];
+    float bound[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
+        valid[k] = false;
+        rb[k] = 0;
+        ms[k] = 0.0f;
+        if (j >= j1) continue;
+        const int64_t i = perm[j];
+        float x[D], v[D];
+        M<D> F, C;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            x[a] = ldf(st, p.stride, L::X + a, i);
+            v[a] = ldf(st, p.stride, L::V + a, i);
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                F.a[a][b] = ldf(st, p.stride, L::F + a * D + b, i);
+                C.a[a][b] = ldf(st, p.stride, L::C + a * D + b, i);
+            }
+        const float Jp = ldf(st, p.stride, L::JP, i);
+        const float mass = ldf(st, p.stride, L::MASS, i);
+        const float vol = ldf(st, p.stride, L::VOL, i);
+        int base[D];
+        float fx[D], w[D][3];
+        if (!stencil<D>(x, p, base, fx, w)) continue;  // keyed into this tile, so never taken
+        valid[k] = true;
+        As[k] = fused_affine<D, MAT>(F, C, Jp, mass, vol, p);
+        ms[k] = mass;
+        float wsum = 1.0f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            fxs[k][a] = fx[a];
+            mvs[k][a] = v[a] * mass;
+            wsum *= fabsf(w[a][0]) + fabsf(w[a][1]) + fabsf(w[a][2]);
+            rb[k] = rb[k] * TL::R + (base[a] - torg[a]);
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            float s = 0.0f;
+#pragma unroll
+            for (int b = 0; b < D; ++b) s += fabsf(As[k].a[c][b]);
+            const float bc = wsum * (fabsf(mvs[k][c]) + 2.5f * p.dx * s);
+            if (bc < 3.0e38f) bound[c] += bc;  // non-finite contributions take the flag path
+        }
+        const float bm = wsum * fabsf(mass);
+        if (bm < 3.0e38f) bound[3] += bm;
+    }
+    // workgroup sum of the bounds -> per channel scale 2^e with bound * 2^e <= 2^61
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float b = bound[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off);
+        bound[c] = b;
+    }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s_bound[threadIdx.x >> 6][c] = bound[c];
+    __syncthreads();
+    float scale[4], inv_scale[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float total = (s_bound[0][c] + s_bound[1][c]) + (s_bound[2][c] + s_bound[3][c]);
+        int ex = 0;
+        (void)frexpf(total > 0.0f && total < 3.0e38f ? total : 1.0f, &ex);  // total < 2^ex
+        const int e = min(127, 61 - ex);                                      // in [-68, 127]
+        scale[c] = ldexpf(1.0f, e);
+        inv_scale[c] = ldexpf(1.0f, -e);
+    }
+
+    // pass 2: quantised contributions -> 64-bit integer LDS atomics
+    constexpr int K = D == 2 ? 9 : 27;
+    constexpr float kLimit = 4.6116860e18f;  // 2^62
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        if (!valid[k]) continue;
+        float w[D][3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) bspline_weights(fxs[k][a], w[a]);
+#pragma unroll
+        for (int o = 0; o < K; ++o) {
+            int off[D];
+            offsets<D>(o, off);
+            float dpos[D];
+            int ro = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                dpos[a] = (float(off[a]) - fxs[k][a]) * p.dx;
+                ro = ro * TL::R + off[a];
+            }
+            float weight = w[0][off[0]];
+#pragma unroll
+            for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
+            const int r = rb[k] + ro;
+            float val[4];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                float ad = As[k].a[c][0] * dpos[0];
+#pragma unroll
+                for (int b = 1; b < D; ++b) ad = ad + As[k].a[c][b] * dpos[b];
+                val[c] = weight * (mvs[k][c] + ad);
+            }
+            if constexpr (D == 2) val[2] = 0.0f;
+            val[3] = weight * ms[k];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (D == 2 && c == 2) continue;
+                const float y = val[c] * scale[c];
+                if (fabsf(y) < kLimit) {
+                    atomicAdd(&acc[c * REG + r], (unsigned long long)(long long)llrintf(y));
+                } else {
+                    atomicOr(&nonfinite[r], 1u << c);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // every work item owns a slot: plain stores, no global atomics
+    float4* dst = slots + int64_t(item) * REG;
+    const float nan = __builtin_nanf("");
+    for (int r = threadIdx.x; r < REG; r += blockDim.x) {
+        float out[4];
+        const uint32_t bad = nonfinite[r];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const long long q = (long long)acc[c * REG + r];
+            out[c] = (bad >> c) & 1u ? nan : float(double(q) * double(inv_scale[c]));
+        }
+        dst[r] = make_float4(out[0], out[1], out[2], out[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// grid update (src/nclr.h:263-310): one workgroup per node tile
+// ---------------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ slots,
+                                                    const uint32_t* __restrict__ item_offset, StepParams p,
+                                                    float4* __restrict__ gv, uint32_t* __restrict__ stamps,
+                                                    uint32_t* __restrict__ active) {
+    using TL = Tile<D>;
+    constexpr int NC = 1 << D;
+    __shared__ int s_first[NC];  // first P2G work item of candidate tile c (its slot), or -1
+    __shared__ int s_count[NC];  // number of work items of candidate tile c
+    __shared__ uint32_t s_active;
+    const uint32_t nt_idx = blockIdx.x;
+    int nto[D];
+    tile_coords<D>(nt_idx, p.ntn, nto);
+    bool occupied = false;
+    if (threadIdx.x == 0) s_active = 0;
+    if (threadIdx.x < NC) {
+        int t = 0;
+        bool valid = true;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int ta = nto[a] - ((threadIdx.x >> a) & 1);
+            valid = valid && ta >= 0 && ta < p.nt;
+            t = t * p.nt + ta;
+        }
+        const uint32_t first = valid ? item_offset[t] : 0u;
+        const uint32_t count = valid ? item_offset[t + 1] - first : 0u;
+        occupied = count > 0;
+        s_first[threadIdx.x] = occupied ? int(first) : -1;
+        s_count[threadIdx.x] = int(count);
+    }
+    if (!__syncthreads_or(occupied)) {
+        if (threadIdx.x == 0) active[nt_idx] = 0;
+        return;
+    }
+    if (threadIdx.x == 0) stamps[nt_idx] = p.stamp;
+    int first[NC], count[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        first[c] = s_first[c];
+        count[c] = s_count[c];
+    }
+    uint32_t massive_nodes = 0;
+    for (int l = threadIdx.x; l < TL::NODES; l += blockDim.x) {
+        int n[D];
+        {
+            int rem = l;
+#pragma unroll
+            for (int a = D - 1; a >= 0; --a) {
+                n[a] = nto[a] * TL::S + (rem & (TL::S - 1));
+                rem >>= TL::LOG;
+            }
+        }
+        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (first[c] < 0) continue;
+            bool inside = true;
+            int r = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const int ra = n[a] - (nto[a] - ((c >> a) & 1)) * TL::S;
+                inside = inside && ra < TL::R;
+                r = r * TL::R + ra;
+            }
+            if (!inside) continue;
+            for (int it = 0; it < count[c]; ++it) {
+                const float4 s = slots[int64_t(first[c] + it) * TL::REGION + r];
+                g.x += s.x;
+                g.y += s.y;
+                g.z += s.z;
+                g.w += s.w;
+            }
+        }
+        float vel[D];
+        vel[0] = g.x;
+        vel[1] = g.y;
+        if constexpr (D == 3) vel[2] = g.z;
+        float m = g.w;
+        massive_nodes += m > 0.0f ? 1u : 0u;
+        if (m > 0.0f) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) vel[a] = vel[a] / m;
+            vel[1] = vel[1] + p.dt * p.gravity;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const float lo = -p.allowed, hi = p.allowed;
+                vel[a] = (vel[a] < lo) ? lo : ((hi < vel[a]) ? hi : vel[a]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            if ((n[a] < 3 && vel[a] < 0.0f) || (n[a] >= (p.res + 1) - 3 && vel[a] > 0.0f)) {
+#pragma unroll
+                for (int b = 0; b < D; ++b) vel[b] = 0.0f;
+                m = 0.0f;
+            }
+        }
+        gv[int64_t(nt_idx) * TL::NODES + l] = make_float4(vel[0], vel[1], D == 3 ? vel[2] : 0.0f, m);
+    }
+    // nodes with post-P2G mass > 0, per node tile (summed on demand by mpm_active_nodes)
+    if (massive_nodes) atomicAdd(&s_active, massive_nodes);
+    __syncthreads();
+    if (threadIdx.x == 0) active[nt_idx] = s_active;
+}
+
+// Sum of n counters into *out (one workgroup).
+__global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t* __restrict__ v, int64_t n,
+                                                  unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[1024];
+    unsigned long long s = 0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) s += v[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// G2P (src/nclr.h:167-261), reading slot perm[j] and writing slot j of the other buffer
+// ---------------------------------------------------------------------------------------------
+template <int D, int MAT>
+__global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st, float* __restrict__ out,
+                                                    const uint32_t* __restrict__ pid_in,
+                                                    uint32_t* __restrict__ pid_out, const uint32_t* __restrict__ perm,
+                                                    StepParams p, const float4* __restrict__ gv) {
+    using L = Layout<D>;
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= p.n) return;
+    const int64_t i = perm[j];
+    const int64_t s = p.stride;
+    pid_out[j] = pid_in[i];
+    float x[D];
+    M<D> F;
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = ldf(st, s, L::X + a, i);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) F.a[a][b] = ldf(st, s, L::F + a * D + b, i);
+    float Jp = ldf(st, s, L::JP, i);
+    out[int64_t(L::MASS) * s + j] = ldf(st, s, L::MASS, i);
+    out[int64_t(L::VOL) * s + j] = ldf(st, s, L::VOL, i);
+    int base[D];
+    float fx[D], w[D][3];
+    if (!stencil<D>(x, p, base, fx, w)) {
+        // outside the grid (already reported by the key kernel): carried through unchanged
+#pragma unroll
+        for (int f = 0; f < L::JP + 1; ++f) out[int64_t(f) * s + j] = ldf(st, s, f, i);
+        return;
+    }
+    float v[D];
+    M<D> C;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        v[a] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < D; ++b) C.a[a][b] = 0.0f;
+    }
+    const float four_inv_dx = 4.0f * p.inv_dx;
+    constexpr int K = D == 2 ? 9 : 27;
+#pragma unroll
+    for (int o = 0; o < K; ++o) {
+        int off[D];
+        offsets<D>(o, off);
+        float dpos[D];
+        int nidx[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            dpos[a] = float(off[a]) - fx[a];
+            nidx[a] = base[a] + off[a];
+        }
+        const float4 g = gv[gv_index<D>(nidx, p.ntn)];
+        const float gvel[3] = {g.x, g.y, g.z};
+        float weight = w[0][off[0]];
+#pragma unroll
+        for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
+#pragma unroll
+        for (int a = 0; a < D; ++a) v[a] = v[a] + weight * gvel[a];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const float wv = four_inv_dx * (weight * gvel[a]);
+#pragma unroll
+            for (int b = 0; b < D; ++b) C.a[a][b] = C.a[a][b] + wv * dpos[b];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = x[a] + p.dt * v[a];
+    M<D> Mx;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            const float id = (a == b && (a < 2 || p.identity_mode == 1)) ? 1.0f : 0.0f;
+            Mx.a[a][b] = id + p.dt * C.a[a][b];
+        }
+    M<D> Fh = matmul<D>(Mx, F);
+    if constexpr (MAT == MPM_JELLY) {
+        F = Fh;
+    } else {
+        M<D> U, V;
+        float sig[D];
+        nclr_svd<D>(Fh, U, sig, V);
+        if constexpr (MAT == MPM_SNOW) {
+            const float lo = float(1.0 - 2.5e-2), hi = float(1.0 + 4.5e-3);
+#pragma unroll
+            for (int d = 0; d < D; ++d) sig[d] = (sig[d] < lo) ? lo : ((hi < sig[d]) ? hi : sig[d]);
+            const float old_J = det(Fh);
+            M<D> US;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) US.a[a][b] = U.a[a][b] * sig[b];
+            const M<D> Fn = matmul_bt<D>(US, V);
+            Jp = Jp * old_J / det(Fn);
+            Jp = (Jp < 0.6f) ? 0.6f : ((20.0f < Jp) ? 20.0f : Jp);
+            F = Fn;
+        } else {
+            double J = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) J *= double(sig[d]);
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) F.a[a][b] = (a == b && (a < 2 || p.identity_mode == 1)) ? 1.0f : 0.0f;
+            F.a[0][0] = float(J);
+        }
+    }
+    out[int64_t(L::JP) * s + j] = Jp;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        out[int64_t(L::X + a) * s + j] = x[a];
+        out[int64_t(L::V + a) * s + j] = v[a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            out[int64_t(L::C + a * D + b) * s + j] = C.a[a][b];
+            out[int64_t(L::F + a * D + b) * s + j] = F.a[a][b];
+        }
+}
+
+// Dense export of the updated grid in the reference's flat x-major order (src/nclr.h:87); nodes of
+// node tiles the last step did not touch are empty cells (zero).
+template <int D>
+__global__ __launch_bounds__(256) void k_export_grid(const float4* __restrict__ gv, const uint32_t* __restrict__ stamps,
+                                                     StepParams p, float* __restrict__ vel,
+                                                     float* __restrict__ mass) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t side = p.res + 1;
+    int64_t total = side * side;
+    if constexpr (D == 3) total *= side;
+    if (idx >= total) return;
+    int n[D];
+    int64_t r = idx;
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) {
+        n[a] = int(r % side);
+        r /= side;
+    }
+    const int64_t node = gv_index<D>(n, p.ntn);
+    const bool live = p.stamp != 0 && stamps[node / Tile<D>::NODES] == p.stamp;
+    const float4 g = live ? gv[node] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    vel[idx * D + 0] = g.x;
+    vel[idx * D + 1] = g.y;
+    if constexpr (D == 3) vel[idx * D + 2] = g.z;
+    mass[idx] = g.w;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t value) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void k_iota_u32(uint32_t* __restrict__ p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = uint32_t(i);
+}
+
+__global__ void k_fill_f4(float4* __restrict__ p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+}  // namespace nclr_dev
+
+// ---------------------------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------------------------
+namespace nclr_host {
+
+using namespace nclr_dev;
+
+static inline unsigned blocks_for(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+static int key_bits(uint32_t max_key) {
+    int b = 0;
+    while (b < 32 && (uint64_t(1) << b) <= max_key) ++b;
+    return b;
+}
+
+template <int D>
+static hipError_t sort_dispatch(const LaunchArgs& a) {
+    const auto& p = a.p;
+    const int64_t nb = int64_t(p.ntiles) + 1;
+    hipStream_t s = a.stream;
+    k_keys<D><<<blocks_for(std::max<int64_t>(p.n, nb), 256), 256, 0, s>>>(a.state_in, p, a.keys, a.status,
+                                                                           a.tile_start, a.tile_end);
+    if (p.n > 0) {
+        hipError_t e = radix_sort_pairs(a.keys, a.perm, a.keys_tmp, a.perm_tmp, a.hist, p.n, key_bits(p.ntiles),
+                                        true, s);
+        if (e != hipSuccess) return e;
+        k_tile_bounds<<<blocks_for(p.n, 256), 256, 0, s>>>(a.keys, p.n, a.tile_start, a.tile_end);
+    }
+    k_items<D><<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, a.item_offset);
+    return hipGetLastError();
+}
+
+template <int D, int MAT>
+static hipError_t p2g_one(const LaunchArgs& a) {
+    const int64_t items = (a.p.n + Tile<D>::CHUNK - 1) / Tile<D>::CHUNK + a.p.ntiles;
+    k_p2g_tile<D, MAT><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.tile_start, a.tile_end,
+                                                              a.item_offset, a.p, a.slots);
+    return hipGetLastError();
+}
+
+template <int D>
+static hipError_t p2g_dispatch(const LaunchArgs& a) {
+    if (a.p.n == 0) return hipSuccess;
+    switch (a.material) {
+        case MPM_SNOW: return p2g_one<D, MPM_SNOW>(a);
+        case MPM_JELLY: return p2g_one<D, MPM_JELLY>(a);
+        default: return p2g_one<D, MPM_LIQUID>(a);
+    }
+}
+
+template <int D, int MAT>
+static hipError_t g2p_one(const LaunchArgs& a) {
+    k_g2p_sorted<D, MAT><<<blocks_for(a.p.n, 256), 256, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in, a.pid_out,
+                                                                       a.perm, a.p, a.gv);
+    return hipGetLastError();
+}
+
+template <int D>
+static hipError_t g2p_dispatch(const LaunchArgs& a) {
+    if (a.p.n == 0) return hipSuccess;
+    switch (a.material) {
+        case MPM_SNOW: return g2p_one<D, MPM_SNOW>(a);
+        case MPM_JELLY: return g2p_one<D, MPM_JELLY>(a);
+        default: return g2p_one<D, MPM_LIQUID>(a);
+    }
+}
+
+hipError_t launch_sort(const LaunchArgs& a) { return a.dim == 2 ? sort_dispatch<2>(a) : sort_dispatch<3>(a); }
+hipError_t launch_p2g(const LaunchArgs& a) { return a.dim == 2 ? p2g_dispatch<2>(a) : p2g_dispatch<3>(a); }
+hipError_t launch_g2p(const LaunchArgs& a) { return a.dim == 2 ? g2p_dispatch<2>(a) : g2p_dispatch<3>(a); }
+
+hipError_t launch_grid(const LaunchArgs& a) {
+    int64_t tiles = 1;
+    for (int d = 0; d < a.dim; ++d) tiles *= a.p.ntn;
+    if (a.dim == 2)
+        k_grid_tiles<2><<<unsigned(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+    else
+        k_grid_tiles<3><<<unsigned(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+    return hipGetLastError();
+}
+
+hipError_t launch_active_sum(const LaunchArgs& a, unsigned long long* out) {
+    int64_t tiles = 1;
+    for (int d = 0; d < a.dim; ++d) tiles *= a.p.ntn;
+    k_sum_u32<<<1, 1024, 0, a.stream>>>(a.active, tiles, out);
+    return hipGetLastError();
+}
+
+int64_t max_items(int dim, int64_t n, uint32_t ntiles) {
+    const int64_t chunk = dim == 2 ? Tile<2>::CHUNK : Tile<3>::CHUNK;
+    return (n + chunk - 1) / chunk + int64_t(ntiles);
+}
+
+hipError_t launch_export_grid(const LaunchArgs& a, float* vel, float* mass, int64_t total) {
+    if (a.dim == 2)
+        k_export_grid<2><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.p, vel, mass);
+    else
+        k_export_grid<3><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.p, vel, mass);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_fill_u32<<<blocks_for(n, 256), 256, 0, s>>>(p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_iota_u32<<<blocks_for(n, 256), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f4(float4* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_fill_f4<<<blocks_for(n, 256), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+int num_fields(int dim) { return dim == 2 ? Layout<2>::NF : Layout<3>::NF; }
+int region_nodes(int dim) { return dim == 2 ? Tile<2>::REGION : Tile<3>::REGION; }
+int tile_nodes(int dim) { return dim == 2 ? Tile<2>::NODES : Tile<3>::NODES; }
+int tile_log(int dim) { return dim == 2 ? Tile<2>::LOG : Tile<3>::LOG; }
+
+}  // namespace nclr_host
