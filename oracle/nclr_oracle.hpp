@@ -356,6 +356,27 @@ class Simulation {
     std::vector<Particle<T, D>>& particles() { return particles_; }
     const std::vector<Cell<T, D>>& grid() const { return cells_; }
     std::vector<Cell<T, D>>& grid_mut() { return cells_; }
+    int res() const { return res_; }
+    T dt() const { return dt_; }
+    T dx() const { return dx_; }
+    T inv_dx() const { return inv_dx_; }
+    T gravity() const { return gravity_; }
+    int material() const { return material_; }
+    int identity_mode() const { return identity_mode_; }
+    T allowed_velocity() const { return T(double(dx_) * 0.9 / double(dt_)); }
+
+    // Flat index of node base+off, checked like cell_at (for callers outside the class).
+    int64_t node_index(const int base[D], const int off[D]) const {
+        int64_t idx = 0;
+        for (int a = 0; a < D; ++a) idx = idx * (res_ + 1) + int64_t(base[a] + off[a]);
+        if (idx < 0 || idx >= num_nodes()) throw OutOfRange("grid index out of range");
+        return idx;
+    }
+
+    template <typename Fn>
+    void each_offset(Fn&& fn) const {
+        for_each_offset(std::forward<Fn>(fn));
+    }
 
     int64_t num_nodes() const {
         int64_t n = 1;

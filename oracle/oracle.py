@@ -218,6 +218,62 @@ def time_steps(state: State, material: int, res: int, dt: float = 1e-4, E: float
     return secs
 
 
+@dataclass
+class Grads:
+    """Cotangents of one step's particle state (and of Young's modulus on the input side)."""
+    x: np.ndarray
+    v: np.ndarray
+    F: np.ndarray
+    C: np.ndarray
+    E: float = 0.0
+
+
+def adjoint_jelly(state: State, grads_out: Grads, res: int = 64, dt: float = 1e-4, E: float = 1e4, nu: float = 0.2,
+                  gravity: float = -100.0, identity_mode: int = 0) -> Grads:
+    """FP64 vector-Jacobian product of one jelly step (nclr_adjoint.hpp): given dL/d(state after the
+    step), returns dL/d(state before the step) and dL/dE."""
+    s = state.astype(np.float64)
+    n, d = s.n, s.dim
+    g = [np.ascontiguousarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")]
+    outs = [np.zeros_like(a) for a in g]
+    e = ctypes.c_double(0.0)
+    lib = _lib()
+    rc = lib.oracle_adjoint_jelly_f64(
+        ctypes.c_int(d), ctypes.c_int(res), ctypes.c_double(dt), ctypes.c_double(E), ctypes.c_double(nu),
+        ctypes.c_double(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(n), _ptr(s.x), _ptr(s.v), _ptr(s.F),
+        _ptr(s.C), _ptr(s.Jp), _ptr(s.mass), _ptr(s.volume), *(_ptr(a) for a in g), *(_ptr(a) for a in outs),
+        ctypes.byref(e))
+    _check(rc, lib)
+    return Grads(*outs, E=e.value)
+
+
+def rollout_vjp_jelly(state: State, grads_out: Grads, steps: int, res: int = 64, dt: float = 1e-4, E: float = 1e4,
+                      nu: float = 0.2, gravity: float = -100.0, identity_mode: int = 0) -> Grads:
+    """FP64 vector-Jacobian product of `steps` jelly steps: the FP64 trajectory is stored and the
+    one-step adjoint is chained backwards through it; dL/dE sums over the steps."""
+    kw = dict(res=res, dt=dt, E=E, nu=nu, gravity=gravity, identity_mode=identity_mode)
+    states = [state.astype(np.float64)]
+    for _ in range(steps - 1):
+        states.append(run(states[-1], JELLY, steps=1, fp64=True, **kw))
+    g = Grads(*(np.asarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")))
+    total_E = 0.0
+    for s in reversed(states):
+        g = adjoint_jelly(s, g, **kw)
+        total_E += g.E
+    g.E = total_E
+    return g
+
+
+def polar_vjp(A: np.ndarray, Rbar: np.ndarray) -> np.ndarray:
+    """FP64 d<Rbar, R(A)>/dA for the rotation factor R of nclr_polar, over a batch (n,d,d)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    Rbar = np.ascontiguousarray(Rbar, dtype=np.float64)
+    n, d, _ = A.shape
+    out = np.empty_like(A)
+    _check(_lib().oracle_polar_vjp_f64(ctypes.c_int(d), ctypes.c_int64(n), _ptr(A), _ptr(Rbar), _ptr(out)), _lib())
+    return out
+
+
 def default_state(x: np.ndarray, identity_mode: int = 0) -> State:
     """Particles as the reference's Particle ctor makes them (src/nclr.h:46-47): v=0, F=diag<dim>(1), C=0,
     Jp=1, mass=volume=1."""

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <string>
 
+#include "nclr_adjoint.hpp"
 #include "nclr_oracle.hpp"
 
 namespace {
@@ -188,6 +189,43 @@ void stress_batch(int material, int res, T dt, T E, T nu, int64_t n, const T* F,
     }
 }
 
+template <int D>
+int adjoint(int res, double dt, double E, double nu, double g, int idm, int64_t n, const double* x,
+            const double* v, const double* F, const double* C, const double* Jp, const double* mass,
+            const double* volume, const double* gx, const double* gv, const double* gF, const double* gC,
+            double* ix, double* iv, double* iF, double* iC, double* iE) {
+    return guarded([&] {
+        const auto ps = load<double, D>(n, x, v, F, C, Jp, mass, volume);
+        oracle::StepGrads<double, D> out(static_cast<size_t>(n)), in;
+        std::copy(gx, gx + n * D, out.x.begin());
+        std::copy(gv, gv + n * D, out.v.begin());
+        std::copy(gF, gF + n * D * D, out.F.begin());
+        std::copy(gC, gC + n * D * D, out.C.begin());
+        oracle::adjoint_step_jelly<double, D>(ps, res, dt, E, nu, g, idm, out, in);
+        std::copy(in.x.begin(), in.x.end(), ix);
+        std::copy(in.v.begin(), in.v.end(), iv);
+        std::copy(in.F.begin(), in.F.end(), iF);
+        std::copy(in.C.begin(), in.C.end(), iC);
+        *iE = in.E;
+    });
+}
+
+template <int D>
+void polar_vjp(int64_t n, const double* A, const double* Rbar, double* Abar) {
+    for (int64_t i = 0; i < n; ++i) {
+        double a[D][D], r[D][D], rb[D][D], fb[D][D] = {};
+        for (int p = 0; p < D; ++p)
+            for (int c = 0; c < D; ++c) {
+                a[p][c] = A[(i * D + p) * D + c];
+                rb[p][c] = Rbar[(i * D + p) * D + c];
+            }
+        oracle::nclr_polar<double, D>(a, r);
+        oracle::polar_backward<double, D>(a, r, rb, fb);
+        for (int p = 0; p < D; ++p)
+            for (int c = 0; c < D; ++c) Abar[(i * D + p) * D + c] = fb[p][c];
+    }
+}
+
 }  // namespace
 
 #define DIM_DISPATCH(dim, call2, call3) \
@@ -240,6 +278,24 @@ const char* oracle_last_error(void) { return g_err.c_str(); }
 
 DEFINE_FOR(float, f32)
 DEFINE_FOR(double, f64)
+
+// FP64 vector-Jacobian product of one jelly step (nclr_adjoint.hpp).
+int oracle_adjoint_jelly_f64(int dim, int res, double dt, double E, double nu, double g, int idm, int64_t n,
+                             const double* x, const double* v, const double* F, const double* C, const double* Jp,
+                             const double* mass, const double* volume, const double* gx, const double* gv,
+                             const double* gF, const double* gC, double* ix, double* iv, double* iF, double* iC,
+                             double* iE) {
+    return DIM_DISPATCH(dim,
+                        (adjoint<2>(res, dt, E, nu, g, idm, n, x, v, F, C, Jp, mass, volume, gx, gv, gF, gC, ix, iv,
+                                    iF, iC, iE)),
+                        (adjoint<3>(res, dt, E, nu, g, idm, n, x, v, F, C, Jp, mass, volume, gx, gv, gF, gC, ix, iv,
+                                    iF, iC, iE)));
+}
+
+// FP64 cotangent of the polar rotation factor: Abar = d<Rbar, R(A)>/dA.
+int oracle_polar_vjp_f64(int dim, int64_t n, const double* A, const double* Rbar, double* Abar) {
+    return DIM_DISPATCH(dim, (polar_vjp<2>(n, A, Rbar, Abar), 0), (polar_vjp<3>(n, A, Rbar, Abar), 0));
+}
 
 // Times `steps` calls of advance() after `warmup` untimed ones on an FP32
 // simulation built from the given particles (the reference's serial CPU

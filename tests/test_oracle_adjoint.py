@@ -1,0 +1,163 @@
+"""The FP64 adjoint of the oracle's jelly step, pinned by central finite differences of the FP64
+oracle step itself (the forward is the restated reference step, src/nclr.h:80-84; the adjoint is
+independent code in oracle/nclr_adjoint.hpp).
+
+Scenes stay away from the grid boundary and the velocity clamp, where the step is smooth: the
+quadratic B-spline weights are C1, the polar factor is smooth for distinct singular values, and
+the clamp / sticky-box switches are inactive.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+KW = dict(res=32, E=1e4, nu=0.2, dt=1e-4)
+
+
+def jelly_state(d, side, seed, F_noise=0.05):
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0.4, 0.6, side)
+    x = np.stack(np.meshgrid(*([g] * d), indexing="ij"), -1).reshape(-1, d)
+    x = x + rng.uniform(-0.01, 0.01, x.shape)
+    s = O.default_state(x).astype(np.float64)
+    n = s.n
+    s.v = rng.normal(0, 0.5, (n, d))
+    s.F = np.eye(d)[None] + rng.normal(0, F_noise, (n, d, d))
+    s.C = rng.normal(0, 2.0, (n, d, d))
+    return s
+
+
+def random_cotangent(s, seed):
+    rng = np.random.default_rng(seed)
+    n, d = s.n, s.dim
+    return O.Grads(x=rng.normal(size=(n, d)), v=rng.normal(size=(n, d)), F=rng.normal(size=(n, d, d)),
+                   C=rng.normal(size=(n, d, d)))
+
+
+def loss(s, G, **kw):
+    o = O.run(s, O.JELLY, fp64=True, steps=1, **kw)
+    return sum(float(np.sum(getattr(G, k) * getattr(o, k))) for k in ("x", "v", "F", "C"))
+
+
+@pytest.mark.parametrize("d,side", [(2, 6), (3, 3)])
+@pytest.mark.parametrize("field", ["x", "v", "F", "C"])
+def test_adjoint_matches_central_differences(d, side, field):
+    s = jelly_state(d, side, seed=0)
+    G = random_cotangent(s, seed=1)
+    gi = O.adjoint_jelly(s, G, **KW)
+    rng = np.random.default_rng(2)
+    flat = getattr(s, field).reshape(-1)
+    fd, an = [], []
+    for i in rng.choice(flat.size, min(10, flat.size), replace=False):
+        h = 1e-6 * max(1.0, abs(flat[i]))
+        sp, sm = s.copy(), s.copy()
+        getattr(sp, field).reshape(-1)[i] += h
+        getattr(sm, field).reshape(-1)[i] -= h
+        fd.append((loss(sp, G, **KW) - loss(sm, G, **KW)) / (2 * h))
+        an.append(getattr(gi, field).reshape(-1)[i])
+    fd, an = np.array(fd), np.array(an)
+    err = np.abs(fd - an).max() / np.abs(fd).max()
+    assert err < 1e-5, (field, err, fd, an)
+
+
+@pytest.mark.parametrize("d,side", [(2, 6), (3, 3)])
+def test_adjoint_E_and_random_direction(d, side):
+    s = jelly_state(d, side, seed=3)
+    G = random_cotangent(s, seed=4)
+    gi = O.adjoint_jelly(s, G, **KW)
+    hE = 1e-2
+    fdE = (loss(s, G, **{**KW, "E": KW["E"] + hE}) - loss(s, G, **{**KW, "E": KW["E"] - hE})) / (2 * hE)
+    assert abs(fdE - gi.E) <= 1e-5 * abs(fdE), (fdE, gi.E)
+    # a joint direction over every field at once
+    rng = np.random.default_rng(5)
+    dirs = {k: rng.normal(size=getattr(s, k).shape) for k in ("x", "v", "F", "C")}
+    h = 1e-7
+    sp, sm = s.copy(), s.copy()
+    for k, u in dirs.items():
+        setattr(sp, k, getattr(s, k) + h * u)
+        setattr(sm, k, getattr(s, k) - h * u)
+    fd = (loss(sp, G, **KW) - loss(sm, G, **KW)) / (2 * h)
+    an = sum(float(np.sum(getattr(gi, k) * u)) for k, u in dirs.items())
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_adjoint_is_linear_in_the_cotangent():
+    s = jelly_state(2, 5, seed=6)
+    G1, G2 = random_cotangent(s, 7), random_cotangent(s, 8)
+    a, b = O.adjoint_jelly(s, G1, **KW), O.adjoint_jelly(s, G2, **KW)
+    Gs = O.Grads(*(2.0 * getattr(G1, k) - 3.0 * getattr(G2, k) for k in ("x", "v", "F", "C")))
+    c = O.adjoint_jelly(s, Gs, **KW)
+    for k in ("x", "v", "F", "C"):
+        np.testing.assert_allclose(getattr(c, k), 2 * getattr(a, k) - 3 * getattr(b, k), rtol=1e-9, atol=1e-9)
+    assert np.isclose(c.E, 2 * a.E - 3 * b.E, rtol=1e-9)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_polar_vjp_matches_central_differences(d):
+    rng = np.random.default_rng(10 + d)
+    A = np.eye(d)[None] + rng.normal(0, 0.3, (6, d, d))
+    A[np.linalg.det(A) < 0] *= -1 if d == 3 else 1
+    Rbar = rng.normal(size=A.shape)
+    got = O.polar_vjp(A, Rbar)
+    h = 1e-6
+    for b in range(A.shape[0]):
+        for i in range(d):
+            for j in range(d):
+                Ap, Am = A[b:b + 1].copy(), A[b:b + 1].copy()
+                Ap[0, i, j] += h
+                Am[0, i, j] -= h
+                fd = (np.sum(Rbar[b] * O.polar(Ap, fp64=True)[0]) - np.sum(Rbar[b] * O.polar(Am, fp64=True)[0])) / (2 * h)
+                assert abs(fd - got[b, i, j]) < 1e-6 * max(1.0, abs(fd)), (b, i, j, fd, got[b, i, j])
+
+
+def test_polar_vjp_of_a_rank_two_3d_matrix():
+    # F = diag(1,1,0)-like deformation gradients (the reference default, Q1) keep a unique polar
+    # rotation, and the backward linear system stays regular
+    rng = np.random.default_rng(20)
+    A = np.diag([1.0, 1.0, 0.0])[None] + rng.normal(0, 0.05, (4, 3, 3))
+    A[:, :, 2] = 0.0
+    Rbar = rng.normal(size=A.shape)
+    got = O.polar_vjp(A, Rbar)
+    h = 1e-6
+    for b in range(A.shape[0]):
+        for i in range(3):
+            for j in range(2):  # stay in the rank-2 set
+                Ap, Am = A[b:b + 1].copy(), A[b:b + 1].copy()
+                Ap[0, i, j] += h
+                Am[0, i, j] -= h
+                fd = (np.sum(Rbar[b] * O.polar(Ap, fp64=True)[0]) - np.sum(Rbar[b] * O.polar(Am, fp64=True)[0])) / (2 * h)
+                assert abs(fd - got[b, i, j]) < 1e-5 * max(1.0, abs(fd)), (b, i, j, fd, got[b, i, j])
+
+
+def test_chained_adjoint_matches_central_differences_over_many_steps():
+    """The multi-step FP64 adjoint (one-step adjoints chained through the stored trajectory)
+    against central differences of the 64-step FP64 rollout: eps = 1e-3 for the initial velocities,
+    C and E (relative), as the differentiable configurations prescribe, and small eps for x and F,
+    whose curvature at a step of 1e-3 (an eighth of a cell) dominates a central difference."""
+    from nuclearmpm_amd import scenes
+
+    d, res, steps, E0 = 2, 128, 64, 5e3
+    x = scenes.cube(d, 20, 0.3, 0.5)
+    s = O.default_state(x).astype(np.float64)
+    rng = np.random.default_rng(0)
+    s.v = (x - x.mean(0)) @ rng.normal(0, 2.0, (d, d)).T + rng.normal(0, 0.1, x.shape)
+    n = s.n
+    G = O.Grads(x=rng.normal(size=(n, d)), v=rng.normal(size=(n, d)), F=rng.normal(size=(n, d, d)),
+                C=np.zeros((n, d, d)))
+    g = O.rollout_vjp_jelly(s, G, steps, res=res, E=E0)
+
+    def L(st, E=E0):
+        o = O.run(st, O.JELLY, res=res, E=E, steps=steps, fp64=True)
+        return sum(float(np.sum(getattr(G, k) * getattr(o, k))) for k in ("x", "v", "F"))
+
+    for k, eps, tol in (("v", 1e-3, 1e-5), ("C", 1e-3, 1e-5), ("x", 1e-7, 1e-5), ("F", 1e-6, 1e-6)):
+        u = rng.normal(size=getattr(s, k).shape)
+        sp, sm = s.copy(), s.copy()
+        setattr(sp, k, getattr(s, k) + eps * u)
+        setattr(sm, k, getattr(s, k) - eps * u)
+        fd = (L(sp) - L(sm)) / (2 * eps)
+        an = float(np.sum(getattr(g, k) * u))
+        assert abs(fd - an) <= tol * abs(an), (k, fd, an)
+    hE = 1e-3 * E0
+    fdE = (L(s, E0 + hE) - L(s, E0 - hE)) / (2 * hE)
+    assert abs(fdE - g.E) <= 1e-4 * abs(fdE), (fdE, g.E)
