@@ -69,6 +69,105 @@ def cpu_baseline(sc, steps: int):
     }
 
 
+def batched_vs_sequential(steps: int):
+    """2D batched workload: 256 seeded scenes (1-4 cubes each, liquid / snow / jelly, res 128) in
+    one launch sequence per step, against the same scenes advanced one plain simulation at a time
+    (both with hipGraph steps); particle-steps/s of each and the ratio."""
+    from nuclearmpm_amd import MPMBatch, MPMSimulation, scenes
+
+    specs = scenes.batched_2d(seed=1, scenes=256)
+    n = sum(s.particles.n for s in specs)
+    with MPMBatch(specs, graph=True) as b:
+        b.advance(5)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.advance(steps)
+        b.synchronize()
+        tb = time.perf_counter() - t0
+    seq_steps = max(5, steps // 4)
+    ts = 0.0
+    for s in specs:
+        with MPMSimulation(s.particles, s.material, res=s.res, dt=s.dt, E=s.E, nu=s.nu, gravity=s.gravity,
+                           graph=True) as sim:
+            sim.advance(2)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.advance(seq_steps)
+            sim.synchronize()
+            ts += time.perf_counter() - t0
+    batched = n * steps / tb
+    sequential = n * seq_steps / ts
+    return {
+        "workload": "batched_2d: 256 scenes (seed 1), 1-4 cubes each, side resolution U{20..80}, "
+                    "materials uniform over liquid/snow/jelly, res 128, dt 1e-4",
+        "particles": n,
+        "batched": {"value": batched, "unit": "particle-steps/s", "steps": steps},
+        "sequential": {"value": sequential, "unit": "particle-steps/s", "steps_per_scene": seq_steps},
+        "speedup": batched / sequential,
+    }
+
+
+def fwd_bwd(which: str):
+    """Forward + backward of a differentiable jelly rollout with checkpoints (torch.autograd over
+    the engine's adjoint kernels); particle-steps/s counts each simulated step once."""
+    import numpy as np
+    import torch
+
+    from nuclearmpm_amd import MPMBatch, MPMSimulation, autodiff, scenes
+
+    if which == "jelly_2d_diff":
+        specs = [scenes.jelly_2d_diff()]
+        sim = MPMSimulation(specs[0].particles, specs[0].material, res=specs[0].res, E=specs[0].E)
+        desc = "jelly_2d_diff: 140x140 particles at [0.3,0.5]^2, res 128, 512 steps, checkpoint every 64"
+    else:
+        specs = scenes.jelly_3d_batched(seed=2, scenes=8)
+        sim = MPMBatch(specs)
+        desc = "jelly_3d_batched: 8 scenes x 64^3 particles, res 128, 256 steps, checkpoint every 32"
+    steps, every = specs[0].steps, int(specs[0].extra["checkpoint_every"])
+    d = specs[0].particles.dim
+    nps = [s.particles.n for s in specs]
+    n = sum(nps)
+    targets = torch.tensor(np.stack([s.extra["target"] for s in specs]), dtype=torch.float64, device="cuda")
+    x0 = torch.cat([autodiff.pack(*(torch.from_numpy(getattr(s.particles, k)).cuda() for k in ("x", "v", "F", "C")))
+                    for s in specs], dim=1)
+    E = torch.tensor([s.E for s in specs], dtype=torch.float64)
+
+    def run():
+        s0 = x0.clone().requires_grad_(True)
+        e = E.clone().requires_grad_(True)
+        out = autodiff.rollout(sim, s0, e, steps, every)
+        xs = autodiff.unpack(out, d)[0]
+        loss = 0.0
+        start = 0
+        for s, m in enumerate(nps):
+            loss = loss + ((xs[start:start + m].double().mean(0) - targets[s]) ** 2).sum()
+            start += m
+        loss.backward()
+        torch.cuda.synchronize()
+        return float(loss), s0.grad, e.grad
+
+    run()  # warm-up (allocations, tape)
+    t0 = time.perf_counter()
+    loss, g, ge = run()
+    t = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        autodiff.rollout(sim, x0, E, steps, every)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    sim.close()
+    return {
+        "workload": desc,
+        "particles": n,
+        "steps": steps,
+        "value": n * steps / t,
+        "unit": "particle-steps/s (forward + backward)",
+        "forward_only": n * steps / tf,
+        "loss": loss,
+        "grad_finite": bool(torch.isfinite(g).all()) and bool(torch.isfinite(ge).all()),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +175,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and forward+backward legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path in this build "
@@ -162,6 +262,15 @@ def main() -> int:
         "g_active": g_active,
         "state_finite": finite,
     }
+    if not args.no_extra:
+        legs = (("batched", lambda: batched_vs_sequential(200)),
+                ("fwd_bwd_2d", lambda: fwd_bwd("jelly_2d_diff")),
+                ("fwd_bwd_3d", lambda: fwd_bwd("jelly_3d_batched")))
+        for name, fn in legs:
+            try:
+                out[name] = fn()
+            except Exception as e:  # report the failing leg without losing the headline number
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
     else:

@@ -1,22 +1,23 @@
 /*
  * nclr_mpm.h - C ABI of the MI355X (gfx950) MLS-MPM engine.
  *
- * The reference engine (jparr721/NuclearMPM) is a header-only C++ class with
- * no FFI; these entry points are what a binding of its step loop needs.  Each
- * declaration names the reference interface it replaces.  The C++ drop-in
- * class in include/nclr/nclr.h and the Python package nuclearmpm_amd are both
- * written on top of this ABI.
+ * The reference engine (jparr721/NuclearMPM) is a header-only C++ class with no FFI; these entry
+ * points are what a binding of its step loop needs.  Each declaration names the reference
+ * interface it replaces (or says that it has none).  The C++ drop-in class in include/nclr/nclr.h
+ * and the Python package nuclearmpm_amd are both written on top of this ABI.
  *
  * Conventions
- *  - Host arrays are particle-major: x[n][dim], v[n][dim], F[n][dim][dim]
- *    (row-major), C[n][dim][dim], Jp[n], mass[n], volume[n].  Device state is
- *    structure-of-arrays inside the library; host buffers are copied in/out.
- *  - Grid arrays use the reference's flat node order, x-major with the last
- *    axis fastest, (res+1)^dim nodes: velocity[node][dim], mass[node].
- *  - Every function returns MPM_OK (0) or an MPM_ERR_* code; mpm_last_error()
- *    gives the message of the last failure on the calling thread.
- *  - One handle owns one HIP stream and must be driven from one host thread
- *    at a time; different handles are independent.
+ *  - Host arrays are particle-major: x[n][dim], v[n][dim], F[n][dim][dim] (row-major),
+ *    C[n][dim][dim], Jp[n], mass[n], volume[n].  Device state is structure-of-arrays inside the
+ *    library; host buffers are copied in/out.
+ *  - Grid arrays use the reference's flat node order, x-major with the last axis fastest,
+ *    (res+1)^dim nodes: velocity[node][dim], mass[node].
+ *  - A handle advances one or more independent scenes (a batch); the particles of scene s are
+ *    rows [sum of earlier scene sizes, + n_s) of every particle array.
+ *  - Every function returns MPM_OK (0) or an MPM_ERR_* code; mpm_last_error() gives the message
+ *    of the last failure on the calling thread.
+ *  - One handle owns one HIP stream and must be driven from one host thread at a time;
+ *    different handles are independent.
  */
 #ifndef NCLR_MPM_H
 #define NCLR_MPM_H
@@ -43,80 +44,171 @@ enum mpm_status {
 
 typedef struct mpm_options {
     /* 0: reference mode, diag<dim>(1) sets (0,0) and (1,1) only (src/nclr_math.h:13-19), so the
-     *    3D "identity" of the F update and the liquid reset is diag(1,1,0) (src/nclr.h:230,256).
-     * 1: physical identity. */
+     *    3D "identity" of the particle ctor, the F update and the liquid reset is diag(1,1,0)
+     *    (src/nclr.h:47,230,256).  1: physical identity. */
     int32_t identity_mode;
-    /* Stable counting sort of the particles by grid block every `sort_every` steps (0 = never). */
-    int32_t sort_every;
-    /* P2G variant: 0 auto, 1 per-particle global atomics, 2 LDS block tiles flushed with atomics. */
-    int32_t p2g_variant;
-    /* Threads per workgroup of the particle kernels (0 = library default). */
-    int32_t block_threads;
     /* HIP device ordinal. */
     int32_t device;
-    /* Record per-kernel HIP events (mpm_kernel_times). */
+    /* Record per-kernel HIP events (mpm_kernel_time). */
     int32_t timing;
-    int32_t reserved[10];
+    /* Particles per P2G work item: 256, 512 or 1024 (0 = 512). */
+    int32_t p2g_chunk;
+    /* G2P workgroup size: 64, 128 or 256 (0 = 256). */
+    int32_t g2p_threads;
+    /* 1: launch each step as one replay of a captured hipGraph (ignored while timing is on). */
+    int32_t graph;
+    /* 1: choose p2g_chunk and g2p_threads at creation by timing trial steps (state restored). */
+    int32_t autotune;
+    /* 3D singular value decompositions (the polar factor in P2G, the snow / liquid SVD in G2P):
+     * 0 = nclr_svd's two-sided Jacobi in Eigen's operation order (src/nclr_math.h:50-74), bitwise
+     * the CPU restatement's FP32 arithmetic; 1 = a fixed-sweep symmetric Jacobi SVD of A^T A with
+     * the same convention (U, V proper rotations, descending order, the sign of det on the last
+     * singular value): cheaper, but its rotation factor carries ~1e-7 absolute error where the
+     * two-sided method is exact to rounding of the (small) strain, which the elastic stress
+     * (F - R) amplifies at small strains. */
+    int32_t fast_svd;
+    int32_t reserved[8];
 } mpm_options;
+
+/* One independent scene of a batch: the reference ctor's arguments (src/nclr.h:74-75). */
+typedef struct mpm_scene {
+    int32_t material;
+    int32_t res;
+    float dt, E, nu, gravity;
+    int64_t n; /* particles of this scene */
+} mpm_scene;
 
 /* Fills *opts with the defaults (reference identity mode, library-chosen launch shapes). */
 void mpm_default_options(mpm_options* opts);
 
 /* Replaces MPMSimulation<dim>::MPMSimulation(particles, model, res, dt, E, nu, gravity)
- * (src/nclr.h:74-78).  Copies the particle arrays to the device.  mass/volume may be NULL
- * (then 1, the Particle default of src/nclr.h:46).  opts may be NULL. */
+ * (src/nclr.h:74-78).  Copies the particle arrays to the device.  Any of v, F, C, Jp, mass,
+ * volume may be NULL for the Particle ctor defaults (src/nclr.h:46-47).  opts may be NULL. */
 int mpm_create(int dim, int material, int res, float dt, float E, float nu, float gravity, int64_t n,
                const float* x, const float* v, const float* F, const float* C, const float* Jp,
                const float* mass, const float* volume, const mpm_options* opts, mpm_handle* out);
 
-/* Replaces nsteps calls of MPMSimulation<dim>::advance() (src/nclr.h:80-84).  The steps are
- * queued on the handle's stream; the call returns without waiting.  Errors raised by queued
- * steps (a particle leaving the grid) are reported by the next synchronising call. */
+/* Many independent simulations advanced by one launch sequence per step.  No reference counterpart
+ * (the reference needs one MPMSimulation object per scene); scene s behaves exactly like
+ * mpm_create with scenes[s]'s arguments on its particle rows. */
+int mpm_create_batch(int dim, int nscenes, const mpm_scene* scenes, const float* x, const float* v,
+                     const float* F, const float* C, const float* Jp, const float* mass, const float* volume,
+                     const mpm_options* opts, mpm_handle* out);
+
+/* Replaces nsteps calls of MPMSimulation<dim>::advance() (src/nclr.h:80-84) for every scene.  The
+ * steps are queued on the handle's stream; the call returns without waiting.  Errors raised by
+ * queued steps (a particle leaving the grid) are reported by the next synchronising call. */
 int mpm_advance(mpm_handle h, int nsteps);
 
 /* Waits for queued work; returns MPM_ERR_OUT_OF_RANGE if any step since the last check found a
  * particle outside the grid (the reference throws std::out_of_range from std::vector::at). */
 int mpm_synchronize(mpm_handle h);
 
-/* Replaces MPMSimulation<dim>::particles() (src/nclr.h:86): copies the current particle state
- * to host arrays in the original particle order.  Any pointer may be NULL to skip that field. */
+/* Replaces MPMSimulation<dim>::particles() (src/nclr.h:86): copies the current particle state of
+ * all scenes to host arrays in the original particle order.  Any pointer may be NULL. */
 int mpm_get_particles(mpm_handle h, float* x, float* v, float* F, float* C, float* Jp, float* mass,
                       float* volume);
 
 /* Replaces the particle vector the constructor takes by value (src/nclr.h:74,76): overwrites the
- * state of all n particles (NULL pointers leave a field unchanged). */
+ * state of all particles (NULL pointers leave a field unchanged). */
 int mpm_set_particles(mpm_handle h, const float* x, const float* v, const float* F, const float* C,
                       const float* Jp, const float* mass, const float* volume);
 
-/* Replaces MPMSimulation<dim>::grid() (src/nclr.h:87): the grid after the last step's grid
- * update, velocity (or momentum where mass == 0) and mass per node, dense flat layout.
+/* Replaces MPMSimulation<dim>::grid() (src/nclr.h:87) for scene 0: the grid after the last step's
+ * grid update, velocity (or momentum where mass == 0) and mass per node, dense flat layout.
  * Before the first step the grid is empty and both arrays are zero-filled. */
 int mpm_get_grid(mpm_handle h, float* velocity, float* mass);
+/* The same for scene `scene` of a batch. */
+int mpm_get_grid_scene(mpm_handle h, int scene, float* velocity, float* mass);
 
-/* Scalars of the simulation: mu_0, lambda_0 (src/nclr.h:71-72), dx, inv_dx. */
+/* Scalars of scene 0: mu_0, lambda_0 (src/nclr.h:71-72), dx, inv_dx. */
 int mpm_get_params(mpm_handle h, float* mu_0, float* lambda_0, float* dx, float* inv_dx);
+int mpm_get_scene_params(mpm_handle h, int scene, float* mu_0, float* lambda_0, float* dx, float* inv_dx);
 int64_t mpm_num_particles(mpm_handle h);
 int64_t mpm_num_nodes(mpm_handle h);
 int64_t mpm_steps_taken(mpm_handle h);
+int mpm_num_scenes(mpm_handle h);
 
-/* Kernel timing (opts.timing = 1): average device milliseconds per launch of each kernel since the
- * last reset, measured with HIP events on the handle's stream, and the launch counts.
- * names: "sort", "p2g", "grid", "g2p".  Returns MPM_ERR_INVALID for an unknown name. */
+/* Kernel timing (timing on): average device milliseconds per launch of each kernel group since
+ * the last reset, measured with HIP events on the handle's stream, and the launch counts.
+ * names: "sort", "p2g", "grid", "g2p", "step" (whole steps, also recorded in graph mode).
+ * Returns MPM_ERR_INVALID for an unknown name. */
 int mpm_kernel_time(mpm_handle h, const char* name, double* avg_ms, int64_t* launches);
 int mpm_reset_kernel_times(mpm_handle h);
 /* Turns the per-kernel HIP event recording of mpm_kernel_time on (1) or off (0). */
 int mpm_set_timing(mpm_handle h, int enable);
 
-/* Number of grid nodes with mass > 0 after the last step's P2G (the G_active count of the
- * roofline bytes). */
+/* Launch shape of the step kernels (engine extras, no reference counterpart). */
+int mpm_set_tuning(mpm_handle h, int p2g_chunk, int g2p_threads);
+int mpm_get_tuning(mpm_handle h, int* p2g_chunk, int* g2p_threads);
+/* Times `steps_per_trial` steps of every launch shape on the current state, keeps the fastest and
+ * restores the state; returns the choice through mpm_get_tuning. */
+int mpm_autotune(mpm_handle h, int steps_per_trial);
+
+/* Number of grid nodes with mass > 0 after the last step's P2G in scene `scene` (-1: all scenes);
+ * the G_active count of the roofline bytes. */
 int mpm_active_nodes(mpm_handle h, int64_t* count);
+int mpm_active_nodes_scene(mpm_handle h, int scene, int64_t* count);
 
 /* The engine's stable counting sort (the per-step particle-by-tile sort) on n host keys:
  * perm[j] = index of the j-th key in stable order by the low `bits` bits (no reference
  * counterpart; the reference steps particles in index order, src/nclr.h:112,169). */
 int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int device);
 
-/* Releases the device memory and stream of h. */
+/* The step's device SVD on n host matrices (dim x dim, row-major): A = U diag(sigma) V^T with
+ * nclr_svd's convention (src/nclr_math.h:50-74: descending order; in 3D U, V proper rotations and
+ * the sign of det A on sigma[2]).  exact = 1: nclr_svd's operation order, 0: the fast path of mpm_options.fast_svd;
+ * 2D always uses it. */
+int mpm_svd(int dim, int64_t n, const float* A, float* U, float* sigma, float* V, int exact, int device);
+
+/* ---- Differentiation (no reference counterpart: the reference has no adjoint) ----------------
+ *
+ * Device-pointer calls.  Every pointer below is device memory on the handle's device; the calls
+ * are stream-ordered on the handle's stream and return before the device finishes: the caller
+ * makes its inputs ready before the call (e.g. synchronises the stream that produced them) and
+ * calls mpm_synchronize before it reads the outputs from another stream.
+ *
+ * State cotangents and device state images are structure-of-arrays over the n particles of the
+ * handle: channel c of particle i at g[c*n + i], with the mpm_grad_channels(dim) = 2 dim + 2 dim^2
+ * channels x[dim], v[dim], F[dim*dim], C[dim*dim] (matrices row-major).  "Particle order" is the
+ * order of the arrays given to mpm_create; "slot order" is the engine's current storage order
+ * (particles are re-sorted by tile every step). */
+int mpm_grad_channels(int dim);
+
+/* Bytes of one state snapshot (all particle fields and the slot -> particle map). */
+int64_t mpm_state_bytes(mpm_handle h);
+/* Copies the current state into / out of a device snapshot of mpm_state_bytes(h) bytes
+ * (checkpoints of a differentiable rollout). */
+int mpm_save_state(mpm_handle h, void* dst);
+int mpm_load_state(mpm_handle h, const void* src);
+
+/* Device image of x, v, F, C in particle order (get), or sets them from one (set; Jp, mass and
+ * volume are kept and the slot order becomes the particle order). */
+int mpm_get_state_device(mpm_handle h, float* dst);
+int mpm_set_state_device(mpm_handle h, const float* src);
+
+/* Re-indexes a cotangent between particle order and the current slot order (src != dst). */
+int mpm_grad_from_particle_order(mpm_handle h, const float* src, float* dst);
+int mpm_grad_to_particle_order(mpm_handle h, const float* src, float* dst);
+
+/* Vector-Jacobian product of one step taken from the current state (jelly scenes only, else
+ * MPM_ERR_UNSUPPORTED): grad_out is the cotangent of the state after the step in ITS slot order,
+ * grad_in (must not alias grad_out) receives the cotangent of the current state in the current
+ * slot order, and grad_E[s] += dL/dE of scene s (nscenes doubles; NULL: not accumulated).  The
+ * state and the step counter are unchanged; the grid export afterwards shows this step's grid. */
+int mpm_backward_step(mpm_handle h, const float* grad_out, float* grad_in, double* grad_E);
+
+/* Vector-Jacobian product of `nsteps` steps from the current state (a checkpoint): recomputes
+ * the states inside the segment on the device, then runs nsteps backward steps in reverse.
+ * grad_out is in the slot order of the state nsteps steps ahead, grad_in in the current slot
+ * order (they may alias).  The current state is left as it was. */
+int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float* grad_in, double* grad_E);
+
+/* Sets Young's modulus and Poisson's ratio of one scene (mu_0, lambda_0 recomputed in float as
+ * src/nclr.h:77-78 does). */
+int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu);
+
+/* Releases the device memory, graphs and stream of h. */
 int mpm_destroy(mpm_handle h);
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -6,14 +6,16 @@ mirror of the reference's C++ API plus the scene generators used by the benchmar
 """
 from ._lib import EngineError, EngineMissing, LIB_PATH, lib
 from .scenes import Scene, XorShift128, cube, linspaced
-from .sim import MaterialModel, MPMSimulation, Particle, Particles, diag
+from .sim import MaterialModel, MPMBatch, MPMSimulation, Particle, Particles, SceneSpec, diag
 
 __all__ = [
     "EngineError",
     "EngineMissing",
     "LIB_PATH",
     "MaterialModel",
+    "MPMBatch",
     "MPMSimulation",
+    "SceneSpec",
     "Particle",
     "Particles",
     "Scene",

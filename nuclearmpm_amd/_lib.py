@@ -27,12 +27,26 @@ MPM_ERR_UNSUPPORTED = 4
 class mpm_options(ctypes.Structure):
     _fields_ = [
         ("identity_mode", ctypes.c_int32),
-        ("sort_every", ctypes.c_int32),
-        ("p2g_variant", ctypes.c_int32),
-        ("block_threads", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("timing", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 10),
+        ("p2g_chunk", ctypes.c_int32),
+        ("g2p_threads", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+        ("fast_svd", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class mpm_scene(ctypes.Structure):
+    _fields_ = [
+        ("material", ctypes.c_int32),
+        ("res", ctypes.c_int32),
+        ("dt", ctypes.c_float),
+        ("E", ctypes.c_float),
+        ("nu", ctypes.c_float),
+        ("gravity", ctypes.c_float),
+        ("n", ctypes.c_int64),
     ]
 
 
@@ -64,6 +78,23 @@ def lib() -> ctypes.CDLL:
     L.mpm_create.argtypes = [i32, i32, i32, f32, f32, f32, f32, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                              ctypes.POINTER(mpm_options), ctypes.POINTER(H)]
     L.mpm_create.restype = i32
+    L.mpm_create_batch.argtypes = [i32, i32, ctypes.POINTER(mpm_scene), _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   ctypes.POINTER(mpm_options), ctypes.POINTER(H)]
+    L.mpm_create_batch.restype = i32
+    L.mpm_get_grid_scene.argtypes = [H, i32, _vp, _vp]
+    L.mpm_get_grid_scene.restype = i32
+    L.mpm_get_scene_params.argtypes = [H, i32, _c_float_p, _c_float_p, _c_float_p, _c_float_p]
+    L.mpm_get_scene_params.restype = i32
+    L.mpm_num_scenes.argtypes = [H]
+    L.mpm_num_scenes.restype = i32
+    L.mpm_set_tuning.argtypes = [H, i32, i32]
+    L.mpm_set_tuning.restype = i32
+    L.mpm_get_tuning.argtypes = [H, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.mpm_get_tuning.restype = i32
+    L.mpm_autotune.argtypes = [H, i32]
+    L.mpm_autotune.restype = i32
+    L.mpm_active_nodes_scene.argtypes = [H, i32, ctypes.POINTER(i64)]
+    L.mpm_active_nodes_scene.restype = i32
     L.mpm_advance.argtypes = [H, i32]
     L.mpm_advance.restype = i32
     L.mpm_synchronize.argtypes = [H]
@@ -92,6 +123,24 @@ def lib() -> ctypes.CDLL:
     L.mpm_active_nodes.restype = i32
     L.mpm_sort_keys.argtypes = [_vp, i64, i32, _vp, i32]
     L.mpm_sort_keys.restype = i32
+    L.mpm_svd.argtypes = [i32, i64, _vp, _vp, _vp, _vp, i32, i32]
+    L.mpm_svd.restype = i32
+    L.mpm_grad_channels.argtypes = [i32]
+    L.mpm_grad_channels.restype = i32
+    L.mpm_state_bytes.argtypes = [H]
+    L.mpm_state_bytes.restype = i64
+    for name in ("mpm_save_state", "mpm_load_state", "mpm_get_state_device", "mpm_set_state_device"):
+        getattr(L, name).argtypes = [H, _vp]
+        getattr(L, name).restype = i32
+    for name in ("mpm_grad_from_particle_order", "mpm_grad_to_particle_order"):
+        getattr(L, name).argtypes = [H, _vp, _vp]
+        getattr(L, name).restype = i32
+    L.mpm_backward_step.argtypes = [H, _vp, _vp, _vp]
+    L.mpm_backward_step.restype = i32
+    L.mpm_backward_segment.argtypes = [H, i32, _vp, _vp, _vp]
+    L.mpm_backward_segment.restype = i32
+    L.mpm_set_scene_elasticity.argtypes = [H, i32, f32, f32]
+    L.mpm_set_scene_elasticity.restype = i32
     L.mpm_destroy.argtypes = [H]
     L.mpm_destroy.restype = i32
     L.mpm_last_error.argtypes = []

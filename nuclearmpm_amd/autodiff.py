@@ -1,0 +1,112 @@
+"""Differentiable jelly rollouts on the HIP engine, as a torch.autograd.Function.
+
+The forward pass advances the engine `steps` steps from a given state, saving a device snapshot
+of the state every `checkpoint_every` steps.  The backward pass walks the segments in reverse: it
+loads a snapshot and calls the engine's segment adjoint (mpm_backward_segment), which recomputes
+the states inside the segment and runs the hand-written adjoint kernels (G2P, grid update, P2G
+with the polar-decomposition backward) step by step in reverse.  Gradients flow to the initial
+x, v, F, C of every particle and to Young's modulus of every scene.
+
+State tensors are (channels, n) float32 on the engine's device: rows x[d], v[d], F[d*d], C[d*d]
+(matrices row-major), particles in the order the simulation was created with.
+"""
+from __future__ import annotations
+
+from typing import Tuple
+
+import torch
+
+from ._lib import check, lib
+from .sim import _Engine
+
+
+def channels(dim: int) -> int:
+    """Rows of a state tensor: x, v, F, C."""
+    return 2 * dim + 2 * dim * dim
+
+
+def pack(x: torch.Tensor, v: torch.Tensor, F: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
+    """(n,d), (n,d), (n,d,d), (n,d,d) -> (channels, n) contiguous float32."""
+    n = x.shape[0]
+    rows = torch.cat([x.reshape(n, -1), v.reshape(n, -1), F.reshape(n, -1), C.reshape(n, -1)], dim=1)
+    return rows.t().contiguous().float()
+
+
+def unpack(state: torch.Tensor, dim: int) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """(channels, n) -> views x (n,d), v (n,d), F (n,d,d), C (n,d,d)."""
+    d = dim
+    t = state.t()
+    n = t.shape[0]
+    return (t[:, :d], t[:, d:2 * d], t[:, 2 * d:2 * d + d * d].reshape(n, d, d),
+            t[:, 2 * d + d * d:].reshape(n, d, d))
+
+
+def _ready(t: torch.Tensor) -> None:
+    # the engine runs on its own stream: inputs produced on torch's stream must be complete
+    torch.cuda.current_stream(t.device).synchronize()
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, state0: torch.Tensor, E: torch.Tensor, sim: _Engine, steps: int, every: int):
+        L, h = lib(), sim._handle()
+        if state0.shape != (channels(sim.dim), sim.n) or state0.dtype != torch.float32 or not state0.is_cuda:
+            raise ValueError("state0 must be a (channels, n) float32 tensor on the engine's device")
+        if E.numel() != sim.scenes:
+            raise ValueError("E needs one value per scene")
+        if steps < 1 or every < 1:
+            raise ValueError("steps and checkpoint_every must be >= 1")
+        state0 = state0.contiguous()
+        _ready(state0)
+        for s, e in enumerate(E.detach().double().cpu().tolist()):
+            sim.set_elasticity(s, e)
+        check(L.mpm_set_state_device(h, state0.data_ptr()))
+        sb = int(L.mpm_state_bytes(h))
+        ckpts = []
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            buf = torch.empty(sb, dtype=torch.uint8, device=state0.device)
+            check(L.mpm_save_state(h, buf.data_ptr()))
+            check(L.mpm_advance(h, k))
+            ckpts.append((buf, k))
+            done += k
+        final = torch.empty(sb, dtype=torch.uint8, device=state0.device)
+        check(L.mpm_save_state(h, final.data_ptr()))
+        out = torch.empty_like(state0)
+        check(L.mpm_get_state_device(h, out.data_ptr()))
+        check(L.mpm_synchronize(h))
+        sim._cache = None
+        ctx.sim = sim
+        ctx.ckpts = ckpts
+        ctx.final = final
+        ctx.E_meta = (E.dtype, E.device, E.shape)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        sim = ctx.sim
+        L, h = lib(), sim._handle()
+        g_user = grad_out.contiguous().float()
+        _ready(g_user)
+        g = torch.empty_like(g_user)
+        check(L.mpm_load_state(h, ctx.final.data_ptr()))
+        check(L.mpm_grad_from_particle_order(h, g_user.data_ptr(), g.data_ptr()))
+        gE = torch.zeros(sim.scenes, dtype=torch.float64, device=g.device)
+        for buf, k in reversed(ctx.ckpts):
+            check(L.mpm_load_state(h, buf.data_ptr()))
+            check(L.mpm_backward_segment(h, int(k), g.data_ptr(), g.data_ptr(), gE.data_ptr()))
+        g0 = torch.empty_like(g)
+        check(L.mpm_grad_to_particle_order(h, g.data_ptr(), g0.data_ptr()))
+        check(L.mpm_synchronize(h))
+        sim._cache = None
+        dtype, device, shape = ctx.E_meta
+        return g0, gE.to(dtype=dtype, device=device).reshape(shape), None, None, None
+
+
+def rollout(sim: _Engine, state0: torch.Tensor, E: torch.Tensor, steps: int, checkpoint_every: int = 32):
+    """Differentiable `steps`-step rollout of a jelly simulation or batch from `state0`
+    ((channels, n) float32 on the engine's device) with Young's modulus E (one per scene).
+    Returns the final state in the same layout.  The simulation is left at the rollout's final
+    state after the forward pass and at its initial state after the backward pass."""
+    return _Rollout.apply(state0, E, sim, int(steps), int(checkpoint_every))

@@ -1,0 +1,717 @@
+// Reverse-mode (adjoint) kernels of one jelly MLS-MPM step for gfx950.
+//
+// Given the cotangent of the state after a step (x', v', F', C') these kernels produce the
+// cotangent of the state before it (x, v, F, C) and of Young's modulus, per scene.  They run after
+// the forward sort, P2G and grid kernels have been recomputed for the state before the step, and
+// reuse that step's tiles, work items and P2G slots:
+//
+//   g2p_adjoint   one workgroup per P2G work item: per particle, the cotangents of the gathered
+//                 velocity and affine matrix are scattered back to the 3^D nodes (exact fixed-point
+//                 sums in LDS, as in P2G, stored per work item); the partial x and F cotangents
+//                 that G2P itself contributes go to grad_in
+//   grid_adjoint  one workgroup per node tile: sums both slot sets, recomputes grid_op
+//                 (src/nclr.h:263-310) and maps the velocity cotangent to the cotangent of the
+//                 pre-update {momentum, mass} (clamped components and sticky nodes pass nothing)
+//   p2g_adjoint   per particle: gathers the grid cotangent through the P2G stencil and the fused
+//                 affine matrix (src/nclr.h:104-165, 313-337), including the backward pass of the
+//                 polar decomposition F = R S: for a cotangent Rbar of R, Fbar = 2 R Y with Y skew
+//                 and S Y + Y S = skew(R^T Rbar); completes grad_in and the per-particle dL/dE
+//   e_sum         per scene, a fixed-order double sum of the dL/dE contributions
+//
+// The same restatement on the CPU in FP64 (the oracle's adjoint) is checked against central
+// finite differences of the step.
+#include <hip/hip_runtime.h>
+
+#include "mpm_device.hpp"
+
+namespace nclr_dev {
+
+// d w_k / d fx of the quadratic B-spline weights (src/nclr.h:124-127)
+NCLR_DI void bspline_dweights(float fx, float (&dw)[3]) {
+    dw[0] = fx - 1.5f;
+    dw[1] = -2.0f * (fx - 1.0f);
+    dw[2] = fx - 0.5f;
+}
+
+// d det(A) / dA
+NCLR_DI M<2> cofactor(const M<2>& a) {
+    M<2> c;
+    c.a[0][0] = a.a[1][1];
+    c.a[0][1] = -a.a[1][0];
+    c.a[1][0] = -a.a[0][1];
+    c.a[1][1] = a.a[0][0];
+    return c;
+}
+
+NCLR_DI M<3> cofactor(const M<3>& a) {
+    M<3> c;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            c.a[i][j] = a.a[i1][j1] * a.a[i2][j2] - a.a[i1][j2] * a.a[i2][j1];
+        }
+    return c;
+}
+
+// Fbar += d<Rbar, R(F)>/dF for the rotation factor R of the polar decomposition F = R S.
+template <int D>
+NCLR_DI void polar_backward(const M<D>& F, const M<D>& R, const M<D>& Rbar, M<D>& Fbar) {
+    M<D> S, Mm;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            float s = 0.0f, m = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                s += R.a[k][i] * F.a[k][j];
+                m += R.a[k][i] * Rbar.a[k][j];
+            }
+            S.a[i][j] = s;
+            Mm.a[i][j] = m;
+        }
+    M<D> Y;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) Y.a[i][j] = 0.0f;
+    if constexpr (D == 2) {
+        const float y = 0.5f * (Mm.a[1][0] - Mm.a[0][1]) / (S.a[0][0] + S.a[1][1]);
+        Y.a[1][0] = y;
+        Y.a[0][1] = -y;
+    } else {
+        // skew(M) as a vector m: (tr(S) I - sym(S)) y = m
+        const float mv[3] = {0.5f * (Mm.a[2][1] - Mm.a[1][2]), 0.5f * (Mm.a[0][2] - Mm.a[2][0]),
+                             0.5f * (Mm.a[1][0] - Mm.a[0][1])};
+        const float tr = S.a[0][0] + S.a[1][1] + S.a[2][2];
+        M<3> Q;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Q.a[i][j] = (i == j ? tr : 0.0f) - 0.5f * (S.a[i][j] + S.a[j][i]);
+        const M<3> Qc = cofactor(Q);
+        const float inv_det = 1.0f / det(Q);
+        float y[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) y[i] = (Qc.a[0][i] * mv[0] + Qc.a[1][i] * mv[1] + Qc.a[2][i] * mv[2]) * inv_det;
+        Y.a[2][1] = y[0];
+        Y.a[1][2] = -y[0];
+        Y.a[0][2] = y[1];
+        Y.a[2][0] = -y[1];
+        Y.a[1][0] = y[2];
+        Y.a[0][1] = -y[2];
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) s += R.a[i][k] * Y.a[k][j];
+            Fbar.a[i][j] += 2.0f * s;
+        }
+}
+
+NCLR_DI float ldg(const float* __restrict__ g, int64_t gstride, int c, int64_t j) { return g[int64_t(c) * gstride + j]; }
+
+// ---------------------------------------------------------------------------------------------
+// G2P adjoint (src/nclr.h:167-234, jelly)
+// ---------------------------------------------------------------------------------------------
+template <int D, int PPT>
+__global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ tile_start,
+                                                     const uint32_t* __restrict__ tile_end,
+                                                     const uint32_t* __restrict__ item_offset, StepParams p,
+                                                     const float4* __restrict__ gv, const float* __restrict__ gout,
+                                                     float* __restrict__ gin, int64_t gstride,
+                                                     float4* __restrict__ slots_b) {
+    using TL = Tile<D>;
+    using L = Layout<D>;
+    constexpr int REG = TL::REGION;
+    constexpr int CHUNK = 256 * PPT;
+    constexpr int K = D == 2 ? 9 : 27;
+    __shared__ unsigned long long acc[D * REG];
+    __shared__ uint32_t nonfinite[REG];
+    __shared__ float s_bound[4][D];
+    __shared__ uint32_t s_tile, s_chunk;
+    const uint32_t item = blockIdx.x;
+    if (item >= item_offset[p.ntiles]) return;
+    if (threadIdx.x == 0) item_tile(item_offset, p.ntiles, item, s_tile, s_chunk);
+    for (int r = threadIdx.x; r < D * REG; r += blockDim.x) acc[r] = 0ull;
+    for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_tile(p, tile)];
+    int torg[D];
+    tile_coords<D>(tile - sp.tile_base, sp.nt, torg);
+#pragma unroll
+    for (int a = 0; a < D; ++a) torg[a] *= TL::S;
+    const uint32_t j0 = tile_start[tile] + s_chunk * CHUNK;
+    const uint32_t j1 = min(j0 + uint32_t(CHUNK), tile_end[tile]);
+    const float four_inv_dx = 4.0f * sp.inv_dx;
+
+    // pass 1: cotangents of the gathered velocity (vb = v' + dt x') and affine matrix
+    // (Cb = C' + dt F' F^T) and a bound of the scattered sums per channel
+    bool valid[PPT];
+    int bases[PPT][D];
+    float fxs[PPT][D], vb[PPT][D];
+    M<D> Cb[PPT];
+    float bound[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) bound[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
+        valid[k] = false;
+        if (j >= j1) continue;
+        const int64_t i = perm[j];
+        float x[D];
+        M<D> F;
+#pragma unroll
+        for (int a = 0; a < D; ++a) x[a] = ldf(st, p.stride, L::X + a, i);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) F.a[a][b] = ldf(st, p.stride, L::F + a * D + b, i);
+        int base[D];
+        float fx[D], w[D][3];
+        if (!stencil<D>(x, sp, base, fx, w)) continue;  // keyed into this tile, so never taken
+        valid[k] = true;
+        float wsum = 1.0f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            bases[k][a] = base[a];
+            fxs[k][a] = fx[a];
+            vb[k][a] = ldg(gout, gstride, L::V + a, j) + sp.dt * ldg(gout, gstride, L::X + a, j);
+            wsum *= fabsf(w[a][0]) + fabsf(w[a][1]) + fabsf(w[a][2]);
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                float s = 0.0f;
+#pragma unroll
+                for (int q = 0; q < D; ++q) s += ldg(gout, gstride, L::F + a * D + q, j) * F.a[b][q];
+                Cb[k].a[a][b] = ldg(gout, gstride, L::C + a * D + b, j) + sp.dt * s;
+            }
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            float s = 0.0f;
+#pragma unroll
+            for (int b = 0; b < D; ++b) s += fabsf(Cb[k].a[a][b]);
+            // |off - fx| <= 1.5 on every axis
+            const float bc = wsum * (fabsf(vb[k][a]) + 1.5f * four_inv_dx * s);
+            if (bc < 3.0e38f) bound[a] += bc;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        float b = bound[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off);
+        bound[c] = b;
+    }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < D; ++c) s_bound[threadIdx.x >> 6][c] = bound[c];
+    __syncthreads();
+    float scale[D], inv_scale[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+        fixed_point_scale((s_bound[0][c] + s_bound[1][c]) + (s_bound[2][c] + s_bound[3][c]), scale[c], inv_scale[c]);
+
+    // pass 2: gather the updated grid (for the forward C and the weight derivatives), scatter the
+    // quantised grid cotangent, write the partial x and F cotangents
+    constexpr float kLimit = 4.6116860e18f;  // 2^62
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        if (!valid[k]) continue;
+        const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
+        float w[D][3], dw[D][3];
+        int rb = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            bspline_weights(fxs[k][a], w[a]);
+            bspline_dweights(fxs[k][a], dw[a]);
+            rb = rb * TL::R + (bases[k][a] - torg[a]);
+        }
+        M<D> Cn;
+        float fxbar[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            fxbar[a] = 0.0f;
+#pragma unroll
+            for (int b = 0; b < D; ++b) Cn.a[a][b] = 0.0f;
+        }
+#pragma unroll
+        for (int o = 0; o < K; ++o) {
+            int off[D];
+            offsets<D>(o, off);
+            float d[D];
+            int nidx[D], ro = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                d[a] = float(off[a]) - fxs[k][a];
+                nidx[a] = bases[k][a] + off[a];
+                ro = ro * TL::R + off[a];
+            }
+            const float4 g = gv[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
+            const float gvel[3] = {g.x, g.y, g.z};
+            float weight = w[0][off[0]];
+#pragma unroll
+            for (int a = 1; a < D; ++a) weight *= w[a][off[a]];
+            float wbar = 0.0f;
+            const int r = rb + ro;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                float cd = 0.0f;
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    cd += Cb[k].a[a][b] * d[b];
+                    Cn.a[a][b] += four_inv_dx * weight * gvel[a] * d[b];
+                }
+                const float val = weight * (vb[k][a] + four_inv_dx * cd);
+                wbar += gvel[a] * (vb[k][a] + four_inv_dx * cd);
+                const float y = val * scale[a];
+                if (fabsf(y) < kLimit) atomicAdd(&acc[a * REG + r], (unsigned long long)(long long)llrintf(y));
+                else atomicOr(&nonfinite[r], 1u << a);
+            }
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                float ctv = 0.0f;
+#pragma unroll
+                for (int a = 0; a < D; ++a) ctv += Cb[k].a[a][b] * gvel[a];
+                fxbar[b] -= four_inv_dx * weight * ctv;
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                float dwa = dw[a][off[a]];
+#pragma unroll
+                for (int b = 0; b < D; ++b)
+                    if (b != a) dwa *= w[b][off[b]];
+                fxbar[a] += wbar * dwa;
+            }
+        }
+        const int64_t i = perm[j];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+            gin[int64_t(L::X + a) * gstride + i] = ldg(gout, gstride, L::X + a, j) + sp.inv_dx * fxbar[a];
+        // F' = (I' + dt C) F  =>  F cotangent (I' + dt C)^T F'
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                float s = 0.0f;
+#pragma unroll
+                for (int q = 0; q < D; ++q) {
+                    const float id1 = (q == a && (a < 2 || sp.identity_mode == 1)) ? 1.0f : 0.0f;
+                    s += (id1 + sp.dt * Cn.a[q][a]) * ldg(gout, gstride, L::F + q * D + b, j);
+                }
+                gin[int64_t(L::F + a * D + b) * gstride + i] = s;
+            }
+    }
+    __syncthreads();
+    float4* dst = slots_b + int64_t(item) * REG;
+    const float nan = __builtin_nanf("");
+    for (int r = threadIdx.x; r < REG; r += blockDim.x) {
+        float out[3] = {0.0f, 0.0f, 0.0f};
+        const uint32_t bad = nonfinite[r];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const long long q = (long long)acc[c * REG + r];
+            out[c] = (bad >> c) & 1u ? nan : float(double(q) * double(inv_scale[c]));
+        }
+        dst[r] = make_float4(out[0], out[1], out[2], 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// grid adjoint (src/nclr.h:263-310)
+// ---------------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ slots,
+                                                      const float4* __restrict__ slots_b,
+                                                      const uint32_t* __restrict__ item_offset, StepParams p,
+                                                      float4* __restrict__ gbar) {
+    using TL = Tile<D>;
+    constexpr int NC = 1 << D;
+    __shared__ int s_first[NC];
+    __shared__ int s_count[NC];
+    const uint32_t nt_idx = blockIdx.x;
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_node_tile(p, nt_idx)];
+    int nto[D];
+    tile_coords<D>(nt_idx - sp.node_tile_base, sp.ntn, nto);
+    bool occupied = false;
+    if (threadIdx.x < NC) {
+        int t = 0;
+        bool valid = true;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int ta = nto[a] - ((threadIdx.x >> a) & 1);
+            valid = valid && ta >= 0 && ta < sp.nt;
+            t = t * sp.nt + ta;
+        }
+        const uint32_t gt = sp.tile_base + uint32_t(t);
+        const uint32_t first = valid ? item_offset[gt] : 0u;
+        const uint32_t count = valid ? item_offset[gt + 1] - first : 0u;
+        occupied = count > 0;
+        s_first[threadIdx.x] = occupied ? int(first) : -1;
+        s_count[threadIdx.x] = int(count);
+    }
+    if (!__syncthreads_or(occupied)) return;  // no particle's stencil reaches this node tile
+    int first[NC], count[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        first[c] = s_first[c];
+        count[c] = s_count[c];
+    }
+    for (int l = threadIdx.x; l < TL::NODES; l += blockDim.x) {
+        int n[D];
+        {
+            int rem = l;
+#pragma unroll
+            for (int a = D - 1; a >= 0; --a) {
+                n[a] = nto[a] * TL::S + (rem & (TL::S - 1));
+                rem >>= TL::LOG;
+            }
+        }
+        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (first[c] < 0) continue;
+            bool inside = true;
+            int r = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const int ra = n[a] - (nto[a] - ((c >> a) & 1)) * TL::S;
+                inside = inside && ra < TL::R;
+                r = r * TL::R + ra;
+            }
+            if (!inside) continue;
+            for (int it = 0; it < count[c]; ++it) {
+                const int64_t slot = int64_t(first[c] + it) * TL::REGION + r;
+                const float4 s = slots[slot];
+                g.x += s.x;
+                g.y += s.y;
+                g.z += s.z;
+                g.w += s.w;
+                const float4 sb = slots_b[slot];
+                b.x += sb.x;
+                b.y += sb.y;
+                b.z += sb.z;
+            }
+        }
+        const float mom[3] = {g.x, g.y, g.z};
+        const float vbar[3] = {b.x, b.y, b.z};
+        const float m = g.w;
+        // forward grid_op, remembering which components were clamped and whether the sticky
+        // boundary zeroed the node
+        float vel[D];
+        bool clamped[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            vel[a] = mom[a];
+            clamped[a] = false;
+        }
+        if (m > 0.0f) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) vel[a] = vel[a] / m;
+            vel[1] = vel[1] + sp.dt * sp.gravity;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                clamped[a] = vel[a] < -sp.allowed || sp.allowed < vel[a];
+                vel[a] = (vel[a] < -sp.allowed) ? -sp.allowed : ((sp.allowed < vel[a]) ? sp.allowed : vel[a]);
+            }
+        }
+        bool sticky = false;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            if ((n[a] < 3 && vel[a] < 0.0f) || (n[a] >= (sp.res + 1) - 3 && vel[a] > 0.0f)) {
+#pragma unroll
+                for (int q = 0; q < D; ++q) vel[q] = 0.0f;
+                sticky = true;
+            }
+        }
+        float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (m > 0.0f && !sticky) {
+            const float inv_m2 = 1.0f / (m * m);
+            float mb = 0.0f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const float vtb = clamped[a] ? 0.0f : vbar[a];
+                out[a] = vtb / m;
+                mb -= vtb * mom[a] * inv_m2;
+            }
+            out[3] = mb;
+        }
+        gbar[int64_t(nt_idx) * TL::NODES + l] = make_float4(out[0], out[1], out[2], out[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// P2G adjoint (src/nclr.h:104-165, 313-337, jelly) per sorted position j
+// ---------------------------------------------------------------------------------------------
+template <int D, bool EXACT>
+__global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ pid,
+                                                     const uint32_t* __restrict__ perm, StepParams p,
+                                                     const float4* __restrict__ gbar, const float* __restrict__ gout,
+                                                     float* __restrict__ gin, int64_t gstride,
+                                                     float* __restrict__ epart) {
+    using L = Layout<D>;
+    constexpr int K = D == 2 ? 9 : 27;
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= p.n) return;
+    const int64_t i = perm[j];
+    const int64_t s = p.stride;
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_particle(p, pid[i])];
+    float x[D], v[D];
+    M<D> F, C;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        x[a] = ldf(st, s, L::X + a, i);
+        v[a] = ldf(st, s, L::V + a, i);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            F.a[a][b] = ldf(st, s, L::F + a * D + b, i);
+            C.a[a][b] = ldf(st, s, L::C + a * D + b, i);
+        }
+    const float Jp = ldf(st, s, L::JP, i);
+    const float mass = ldf(st, s, L::MASS, i);
+    const float vol = ldf(st, s, L::VOL, i);
+    int base[D];
+    float fx[D], w[D][3];
+    if (!stencil<D>(x, sp, base, fx, w)) {
+        // outside the grid the forward step carries the particle through unchanged
+        for (int c = 0; c < L::NG; ++c) gin[int64_t(c) * gstride + i] = ldg(gout, gstride, c, j);
+        epart[j] = 0.0f;
+        return;
+    }
+    float dw[D][3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) bspline_dweights(fx[a], dw[a]);
+    const M<D> A = fused_affine<D, EXACT>(MPM_JELLY, F, C, Jp, mass, vol, sp);
+    M<D> Abar;
+    float fxbar[D], vin[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        fxbar[a] = 0.0f;
+        vin[a] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < D; ++b) Abar.a[a][b] = 0.0f;
+    }
+#pragma unroll
+    for (int o = 0; o < K; ++o) {
+        int off[D];
+        offsets<D>(o, off);
+        float dpos[D];
+        int nidx[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            dpos[a] = (float(off[a]) - fx[a]) * sp.dx;
+            nidx[a] = base[a] + off[a];
+        }
+        const float4 g = gbar[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
+        const float gm[3] = {g.x, g.y, g.z};
+        float weight = w[0][off[0]];
+#pragma unroll
+        for (int a = 1; a < D; ++a) weight *= w[a][off[a]];
+        float wbar = g.w * mass;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            float ad = 0.0f;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                ad += A.a[r][c] * dpos[c];
+                Abar.a[r][c] += weight * gm[r] * dpos[c];
+            }
+            wbar += gm[r] * (mass * v[r] + ad);
+            vin[r] += weight * mass * gm[r];
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            float atm = 0.0f;
+#pragma unroll
+            for (int r = 0; r < D; ++r) atm += A.a[r][c] * gm[r];
+            fxbar[c] -= sp.dx * weight * atm;
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            float dwa = dw[a][off[a]];
+#pragma unroll
+            for (int b = 0; b < D; ++b)
+                if (b != a) dwa *= w[b][off[b]];
+            fxbar[a] += wbar * dwa;
+        }
+    }
+    // A = -(dt V0) Dinv PF + m C  ->  G = dL/dPF
+    const float Dinv = 4.0f * sp.inv_dx * sp.inv_dx;
+    const float e = hardening(MPM_JELLY, Jp);
+    const float mu = sp.mu0 * e, lambda = sp.lambda0 * e;
+    M<D> G;
+    float sG = 0.0f;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            G.a[a][b] = -(sp.dt * vol) * Dinv * Abar.a[a][b];
+            sG += G.a[a][b];
+        }
+    // PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry
+    const M<D> R = polar_rotation_mode<EXACT>(F);
+    const float J = det(F);
+    M<D> Fbar, Rbar;
+    float mubar = 0.0f;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            float gf = 0.0f, gtfr = 0.0f, fmrf = 0.0f;
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                gf += G.a[a][q] * F.a[q][b];
+                gtfr += G.a[q][a] * (F.a[q][b] - R.a[q][b]);
+                fmrf += (F.a[a][q] - R.a[a][q]) * F.a[b][q];
+            }
+            Fbar.a[a][b] = 2.0f * mu * (gf + gtfr);
+            Rbar.a[a][b] = -2.0f * mu * gf;
+            mubar += G.a[a][b] * 2.0f * fmrf;
+        }
+    polar_backward<D>(F, R, Rbar, Fbar);
+    const M<D> cof = cofactor(F);
+    const float Jbar = sG * lambda * (2.0f * J - 1.0f);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        gin[int64_t(L::X + a) * gstride + i] += sp.inv_dx * fxbar[a];
+        gin[int64_t(L::V + a) * gstride + i] = vin[a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            gin[int64_t(L::F + a * D + b) * gstride + i] += Fbar.a[a][b] + Jbar * cof.a[a][b];
+            gin[int64_t(L::C + a * D + b) * gstride + i] = mass * Abar.a[a][b];
+        }
+    const float lambar = sG * (J - 1.0f) * J;
+    epart[j] = e * (mubar * sp.dmu_dE + lambar * sp.dlambda_dE);
+}
+
+// grad_E[s] += sum of epart over scene s's sorted particles, in a fixed order (one workgroup per
+// scene; scenes occupy consecutive sorted keys).
+__global__ __launch_bounds__(256) void k_scene_e_sum(const uint32_t* __restrict__ keys, const float* __restrict__ epart,
+                                                     StepParams p, double* __restrict__ grad_E) {
+    __shared__ double part[256];
+    __shared__ int64_t s_lo, s_hi;
+    const int sc = blockIdx.x;
+    if (threadIdx.x < 2) {
+        const uint32_t key = threadIdx.x == 0 ? p.scenes[sc].tile_base
+                                              : (sc + 1 < p.nscenes ? p.scenes[sc + 1].tile_base : p.ntiles);
+        int64_t lo = 0, hi = p.n;  // first sorted position with keys[.] >= key
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        if (threadIdx.x == 0) s_lo = lo;
+        else s_hi = lo;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int64_t j = s_lo + threadIdx.x; j < s_hi; j += 256) acc += double(epart[j]);
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad_E[sc] += part[0];
+}
+
+// dst[c][to(i)] = src[c][from(i)]: with to_particle_order, slot i goes to particle pid[i];
+// otherwise slot i takes particle pid[i].
+__global__ __launch_bounds__(256) void k_permute_channels(const uint32_t* __restrict__ pid, const float* __restrict__ src,
+                                                          int64_t src_stride, float* __restrict__ dst,
+                                                          int64_t dst_stride, int channels, int64_t n,
+                                                          int to_particle_order) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t u = pid[i];
+    const int64_t si = to_particle_order ? i : u;
+    const int64_t di = to_particle_order ? u : i;
+    for (int c = 0; c < channels; ++c) dst[int64_t(c) * dst_stride + di] = src[int64_t(c) * src_stride + si];
+}
+
+}  // namespace nclr_dev
+
+namespace nclr_host {
+
+using namespace nclr_dev;
+
+static inline unsigned blocks_of(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+template <int D, int PPT>
+static hipError_t g2p_adjoint_one(const LaunchArgs& a, const AdjointArgs& b) {
+    const int64_t items = (a.p.n + 256 * PPT - 1) / (256 * PPT) + a.p.ntiles;
+    k_g2p_adjoint<D, PPT><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.tile_start, a.tile_end,
+                                                                 a.item_offset, a.p, a.gv, b.grad_out, b.grad_in,
+                                                                 b.gstride, b.slots_b);
+    return hipGetLastError();
+}
+
+template <int D>
+static hipError_t g2p_adjoint_dispatch(const LaunchArgs& a, const AdjointArgs& b) {
+    switch (a.p.chunk) {
+        case 256: return g2p_adjoint_one<D, 1>(a, b);
+        case 1024: return g2p_adjoint_one<D, 4>(a, b);
+        default: return g2p_adjoint_one<D, 2>(a, b);
+    }
+}
+
+hipError_t launch_g2p_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
+    if (a.p.n == 0) return hipSuccess;
+    return a.dim == 2 ? g2p_adjoint_dispatch<2>(a, b) : g2p_adjoint_dispatch<3>(a, b);
+}
+
+hipError_t launch_grid_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
+    const unsigned tiles = a.p.node_tiles;
+    if (tiles == 0 || a.p.n == 0) return hipSuccess;
+    if (a.dim == 2)
+        k_grid_adjoint<2><<<tiles, 256, 0, a.stream>>>(a.slots, b.slots_b, a.item_offset, a.p, b.gbar);
+    else
+        k_grid_adjoint<3><<<tiles, 256, 0, a.stream>>>(a.slots, b.slots_b, a.item_offset, a.p, b.gbar);
+    return hipGetLastError();
+}
+
+hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
+    if (a.p.n > 0) {
+        if (a.dim == 2)
+            k_p2g_adjoint<2, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
+                a.state_in, a.pid_in, a.perm, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
+        else if (a.exact_svd)
+            k_p2g_adjoint<3, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
+                a.state_in, a.pid_in, a.perm, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
+        else
+            k_p2g_adjoint<3, false><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
+                a.state_in, a.pid_in, a.perm, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (a.p.n > 0 && b.grad_E) k_scene_e_sum<<<unsigned(a.p.nscenes), 256, 0, a.stream>>>(a.keys, b.epart, a.p, b.grad_E);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute_channels(const uint32_t* pid, const float* src, int64_t src_stride, float* dst,
+                                   int64_t dst_stride, int channels, int64_t n, bool to_particle_order,
+                                   hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_permute_channels<<<blocks_of(n, 256), 256, 0, s>>>(pid, src, src_stride, dst, dst_stride, channels, n,
+                                                         to_particle_order ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace nclr_host

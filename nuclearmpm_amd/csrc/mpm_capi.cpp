@@ -1,11 +1,13 @@
 // C ABI of the MLS-MPM engine (declarations and contracts: include/nclr_mpm.h).
 //
-// One handle = one simulation = one HIP stream.  The handle owns two ping-pong buffers of
-// structure-of-arrays particle state (G2P writes the state in tile order into the other buffer),
-// the original particle index of every slot, the sort buffers, the per-tile P2G slots, the
-// tile-blocked updated grid and a device status word that kernels set when a particle leaves
-// the grid.  The status is read back (and the error reported) only by synchronising calls, so
-// mpm_advance() never waits on the device.
+// One handle = one batch of scenes (a single simulation is a batch of one) = one HIP stream.
+// The handle owns two ping-pong buffers of structure-of-arrays particle state (G2P writes the
+// state in tile order into the other buffer), the original particle index of every slot, the
+// sort buffers, the per-work-item P2G slots, the tile-blocked updated grid, the device step
+// stamp, and a device status word that kernels set when a particle leaves the grid.  The status
+// is read back (and the error reported) only by synchronising calls, so mpm_advance() never
+// waits on the device.  With opts.graph each step is one replay of a captured hipGraph (one
+// graph per ping-pong parity; every other kernel argument is step-invariant).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,8 +33,26 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(MPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum KernelId { kSort = 0, kP2G = 1, kGrid = 2, kG2P = 3, kNumKernels = 4 };
-const char* const kKernelNames[kNumKernels] = {"sort", "p2g", "grid", "g2p"};
+#define RC_TRY(expr)                       \
+    do {                                   \
+        const int rc_ = (expr);            \
+        if (rc_ != MPM_OK) return rc_;     \
+    } while (0)
+
+enum KernelId {
+    kSort = 0,
+    kP2G,
+    kGrid,
+    kG2P,
+    kStep,
+    kG2PAdj,
+    kGridAdj,
+    kP2GAdj,
+    kBackward,
+    kNumKernels
+};
+const char* const kKernelNames[kNumKernels] = {"sort",        "p2g",          "grid",        "g2p",     "step",
+                                               "g2p_adjoint", "grid_adjoint", "p2g_adjoint", "backward"};
 
 struct PendingEvent {
     int kernel;
@@ -49,16 +69,23 @@ hipError_t dmalloc(T** p, size_t count) {
     return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
 }
 
+bool valid_chunk(int c) { return c == 256 || c == 512 || c == 1024; }
+bool valid_threads(int t) { return t == 64 || t == 128 || t == 256; }
+
 }  // namespace
 
 struct nclr_mpm_sim {
-    int dim = 0, material = 0, res = 0;
+    int dim = 0;
+    int material = 0;  // common material of all scenes or nclr_dev::kMixed
     mpm_options opts{};
+    std::vector<nclr_dev::SceneParams> scenes;
+    nclr_dev::SceneParams* d_scenes = nullptr;
     nclr_dev::StepParams p{};
     hipStream_t stream = nullptr;
     float* state[2] = {nullptr, nullptr};
     uint32_t* pid[2] = {nullptr, nullptr};
     int cur = 0;
+    uint32_t* stamp = nullptr;  // device step stamp
     uint32_t *keys = nullptr, *perm = nullptr, *keys_tmp = nullptr, *perm_tmp = nullptr, *hist = nullptr;
     uint32_t *tile_start = nullptr, *tile_end = nullptr, *item_offset = nullptr;
     float4* slots = nullptr;
@@ -67,21 +94,32 @@ struct nclr_mpm_sim {
     uint32_t* status = nullptr;
     uint32_t* active = nullptr;
     unsigned long long* active_sum = nullptr;
-    int64_t node_tiles = 0;
+    // adjoint buffers, allocated by the first backward call
+    float4* slots_b = nullptr;
+    float4* gbar = nullptr;
+    float* epart = nullptr;
+    float* gtmp[2] = {nullptr, nullptr};
+    char* tape = nullptr;  // saved states of a backward segment
+    int64_t tape_states = 0;
+    int g2p_threads = 256;
     int64_t steps = 0;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     std::vector<PendingEvent> pending;
     std::vector<hipEvent_t> pool;
     KernelTimer timers[kNumKernels];
 
-    nclr_host::LaunchArgs args() const {
+    nclr_host::LaunchArgs args(int parity) const {
         nclr_host::LaunchArgs a{};
         a.dim = dim;
         a.material = material;
+        a.g2p_threads = g2p_threads;
+        a.exact_svd = opts.fast_svd ? 0 : 1;
         a.p = p;
-        a.state_in = state[cur];
-        a.state_out = state[cur ^ 1];
-        a.pid_in = pid[cur];
-        a.pid_out = pid[cur ^ 1];
+        a.stamp = stamp;
+        a.state_in = state[parity];
+        a.state_out = state[parity ^ 1];
+        a.pid_in = pid[parity];
+        a.pid_out = pid[parity ^ 1];
         a.keys = keys;
         a.perm = perm;
         a.keys_tmp = keys_tmp;
@@ -100,9 +138,9 @@ struct nclr_mpm_sim {
     }
 
     int nf() const { return nclr_host::num_fields(dim); }
-    int64_t num_nodes() const {
+    int64_t scene_nodes(int s) const {
         int64_t n = 1;
-        for (int a = 0; a < dim; ++a) n *= (res + 1);
+        for (int a = 0; a < dim; ++a) n *= (scenes[size_t(s)].res + 1);
         return n;
     }
 
@@ -132,37 +170,77 @@ struct nclr_mpm_sim {
         return MPM_OK;
     }
 
+    int record_start(PendingEvent& pe, int kernel) {
+        if (pending.size() > 8192) RC_TRY(resolve_events());
+        pe = PendingEvent{kernel, take_event(), take_event()};
+        if (!pe.start || !pe.stop) return fail(MPM_ERR_HIP, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(pe.start, stream));
+        return MPM_OK;
+    }
+    int record_stop(PendingEvent& pe) {
+        HIP_TRY(hipEventRecord(pe.stop, stream));
+        pending.push_back(pe);
+        return MPM_OK;
+    }
+
     template <typename Fn>
     int timed(int kernel, Fn&& launch) {
         if (!opts.timing) {
             HIP_TRY(launch());
             return MPM_OK;
         }
-        if (pending.size() > 8192) {
-            const int rc = resolve_events();
-            if (rc != MPM_OK) return rc;
-        }
-        PendingEvent pe{kernel, take_event(), take_event()};
-        if (!pe.start || !pe.stop) return fail(MPM_ERR_HIP, "hipEventCreate failed");
-        HIP_TRY(hipEventRecord(pe.start, stream));
+        PendingEvent pe{};
+        RC_TRY(record_start(pe, kernel));
         HIP_TRY(launch());
-        HIP_TRY(hipEventRecord(pe.stop, stream));
-        pending.push_back(pe);
+        return record_stop(pe);
+    }
+
+    int launch_step_kernels(const nclr_host::LaunchArgs& a) {
+        RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
+        RC_TRY(timed(kP2G, [&] { return nclr_host::launch_p2g(a); }));
+        RC_TRY(timed(kGrid, [&] { return nclr_host::launch_grid(a); }));
+        RC_TRY(timed(kG2P, [&] { return nclr_host::launch_g2p(a); }));
+        return MPM_OK;
+    }
+
+    void drop_graphs() {
+        for (auto& g : graph_exec)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+
+    int capture(int parity) {
+        const nclr_host::LaunchArgs a = args(parity);
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = nclr_host::launch_sort(a);
+        if (e == hipSuccess) e = nclr_host::launch_p2g(a);
+        if (e == hipSuccess) e = nclr_host::launch_grid(a);
+        if (e == hipSuccess) e = nclr_host::launch_g2p(a);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream, &graph);
+        if (e != hipSuccess || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(MPM_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+        }
+        e = hipGraphInstantiate(&graph_exec[parity], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         return MPM_OK;
     }
 
     int step() {
-        p.stamp = p.stamp + 1;
-        if (p.stamp == 0) p.stamp = 1;
-        const nclr_host::LaunchArgs a = args();
-        int rc = timed(kSort, [&] { return nclr_host::launch_sort(a); });
-        if (rc != MPM_OK) return rc;
-        rc = timed(kP2G, [&] { return nclr_host::launch_p2g(a); });
-        if (rc != MPM_OK) return rc;
-        rc = timed(kGrid, [&] { return nclr_host::launch_grid(a); });
-        if (rc != MPM_OK) return rc;
-        rc = timed(kG2P, [&] { return nclr_host::launch_g2p(a); });
-        if (rc != MPM_OK) return rc;
+        const bool use_graph = opts.graph && !opts.timing;
+        if (use_graph) {
+            if (!graph_exec[cur]) RC_TRY(capture(cur));
+            HIP_TRY(hipGraphLaunch(graph_exec[cur], stream));
+        } else {
+            PendingEvent pe{};
+            if (opts.timing) RC_TRY(record_start(pe, kStep));
+            RC_TRY(launch_step_kernels(args(cur)));
+            if (opts.timing) RC_TRY(record_stop(pe));
+        }
         if (p.n > 0) cur ^= 1;
         steps += 1;
         return MPM_OK;
@@ -173,10 +251,7 @@ struct nclr_mpm_sim {
         HIP_TRY(hipStreamSynchronize(stream));
         uint32_t st = 0;
         HIP_TRY(hipMemcpy(&st, status, sizeof(st), hipMemcpyDeviceToHost));
-        if (opts.timing) {
-            const int rc = resolve_events();
-            if (rc != MPM_OK) return rc;
-        }
+        if (opts.timing) RC_TRY(resolve_events());
         if (st != 0) {
             HIP_TRY(nclr_host::launch_fill_u32(status, 1, 0u, stream));
             HIP_TRY(hipStreamSynchronize(stream));
@@ -235,43 +310,247 @@ struct nclr_mpm_sim {
         host.swap(out);
     }
 
+    int set_tuning(int chunk, int threads) {
+        chunk = chunk ? chunk : 512;
+        threads = threads ? threads : 256;
+        if (!valid_chunk(chunk)) return fail(MPM_ERR_INVALID, "p2g_chunk must be 256, 512 or 1024");
+        if (!valid_threads(threads)) return fail(MPM_ERR_INVALID, "g2p_threads must be 64, 128 or 256");
+        if (chunk != p.chunk || threads != g2p_threads) {
+            HIP_TRY(hipStreamSynchronize(stream));
+            drop_graphs();
+        }
+        p.chunk = chunk;
+        g2p_threads = threads;
+        return MPM_OK;
+    }
+
+    int autotune(int trial_steps) {
+        if (trial_steps < 1) return fail(MPM_ERR_INVALID, "steps_per_trial must be >= 1");
+        RC_TRY(sync());
+        const size_t state_bytes = size_t(nf()) * size_t(p.stride) * sizeof(float);
+        const size_t pid_bytes = size_t(p.stride) * sizeof(uint32_t);
+        float* saved = nullptr;
+        uint32_t* saved_pid = nullptr;
+        HIP_TRY(dmalloc(&saved, state_bytes / sizeof(float)));
+        if (dmalloc(&saved_pid, size_t(p.stride)) != hipSuccess) {
+            (void)hipFree(saved);
+            return fail(MPM_ERR_HIP, "hipMalloc failed");
+        }
+        const int cur0 = cur;
+        const int64_t steps0 = steps;
+        const int timing0 = opts.timing;
+        opts.timing = 0;
+        hipEvent_t e0 = take_event(), e1 = take_event();
+        int rc = (e0 && e1) ? MPM_OK : fail(MPM_ERR_HIP, "hipEventCreate failed");
+        auto restore = [&]() -> int {
+            cur = cur0;
+            HIP_TRY(hipMemcpyAsync(state[cur], saved, state_bytes, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(pid[cur], saved_pid, pid_bytes, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            return MPM_OK;
+        };
+        if (rc == MPM_OK && hipMemcpyAsync(saved, state[cur], state_bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+            rc = fail(MPM_ERR_HIP, "autotune: state copy failed");
+        if (rc == MPM_OK && hipMemcpyAsync(saved_pid, pid[cur], pid_bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+            rc = fail(MPM_ERR_HIP, "autotune: state copy failed");
+        int best_chunk = p.chunk, best_threads = g2p_threads;
+        float best_ms = 3.0e38f;
+        const int chunks[3] = {256, 512, 1024};
+        const int threads[3] = {64, 128, 256};
+        for (int ci = 0; ci < 3 && rc == MPM_OK; ++ci)
+            for (int ti = 0; ti < 3 && rc == MPM_OK; ++ti) {
+                rc = restore();
+                if (rc == MPM_OK) rc = set_tuning(chunks[ci], threads[ti]);
+                if (rc == MPM_OK) rc = step();  // warm-up (and graph capture)
+                if (rc == MPM_OK && hipEventRecord(e0, stream) != hipSuccess) rc = fail(MPM_ERR_HIP, "event");
+                for (int s = 0; s < trial_steps && rc == MPM_OK; ++s) rc = step();
+                if (rc == MPM_OK && hipEventRecord(e1, stream) != hipSuccess) rc = fail(MPM_ERR_HIP, "event");
+                float ms = 0.0f;
+                if (rc == MPM_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = fail(MPM_ERR_HIP, "autotune: event timing failed");
+                if (rc == MPM_OK && ms < best_ms) {
+                    best_ms = ms;
+                    best_chunk = chunks[ci];
+                    best_threads = threads[ti];
+                }
+            }
+        if (rc == MPM_OK) rc = restore();
+        if (rc == MPM_OK) rc = set_tuning(best_chunk, best_threads);
+        // trial steps may have set the out-of-range flag for states that leave the grid; the real
+        // steps will report it again
+        if (rc == MPM_OK) rc = nclr_host::launch_fill_u32(status, 1, 0u, stream) == hipSuccess ? MPM_OK : fail(MPM_ERR_HIP, "status reset");
+        steps = steps0;
+        opts.timing = timing0;
+        if (e0) pool.push_back(e0);
+        if (e1) pool.push_back(e1);
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(saved);
+        (void)hipFree(saved_pid);
+        return rc;
+    }
+
+    // ---- state snapshots and the adjoint -------------------------------------------------------
+    int grad_channels() const { return 2 * dim + 2 * dim * dim; }
+    size_t state_floats() const { return size_t(nf()) * size_t(p.stride); }
+    size_t state_bytes() const { return state_floats() * sizeof(float) + size_t(p.stride) * sizeof(uint32_t); }
+
+    // Snapshot layout: the SoA state (nf fields x stride floats), then the slot -> particle map.
+    int save_state(void* dst) {
+        char* d = static_cast<char*>(dst);
+        HIP_TRY(hipMemcpyAsync(d, state[cur], state_floats() * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d + state_floats() * sizeof(float), pid[cur], size_t(p.stride) * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, stream));
+        return MPM_OK;
+    }
+    int load_state(const void* src) {
+        const char* s = static_cast<const char*>(src);
+        HIP_TRY(hipMemcpyAsync(state[cur], s, state_floats() * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(pid[cur], s + state_floats() * sizeof(float), size_t(p.stride) * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, stream));
+        return MPM_OK;
+    }
+
+    int ensure_adjoint() {
+        if (slots_b) return MPM_OK;
+        const size_t slot_nodes = size_t(nclr_host::max_items(p.n, p.ntiles)) * nclr_host::region_nodes(dim);
+        const size_t gv_nodes = size_t(p.node_tiles) * nclr_host::tile_nodes(dim);
+        const size_t gfl = size_t(grad_channels()) * size_t(std::max<int64_t>(p.n, 1));
+        if (dmalloc(&slots_b, slot_nodes) != hipSuccess || dmalloc(&gbar, gv_nodes) != hipSuccess ||
+            dmalloc(&epart, size_t(std::max<int64_t>(p.n, 1))) != hipSuccess ||
+            dmalloc(&gtmp[0], gfl) != hipSuccess || dmalloc(&gtmp[1], gfl) != hipSuccess)
+            return fail(MPM_ERR_HIP, "hipMalloc of the adjoint buffers failed (out of device memory?)");
+        return MPM_OK;
+    }
+
+    int check_differentiable() const {
+        for (const auto& sp : scenes)
+            if (sp.material != MPM_JELLY)
+                return fail(MPM_ERR_UNSUPPORTED, "the adjoint step is implemented for jelly scenes only");
+        return MPM_OK;
+    }
+
+    // One adjoint step of the current state: recompute sort, P2G and the grid update, then run the
+    // adjoint kernels.  The state and the step counter are left unchanged.
+    int backward_step(const float* gout, float* gin, double* grad_E) {
+        RC_TRY(check_differentiable());
+        RC_TRY(ensure_adjoint());
+        if (p.n == 0) return MPM_OK;
+        const nclr_host::LaunchArgs a = args(cur);
+        const nclr_host::AdjointArgs b{gout, gin, p.n, slots_b, gbar, epart, grad_E};
+        PendingEvent pe{};
+        if (opts.timing) RC_TRY(record_start(pe, kBackward));
+        RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
+        RC_TRY(timed(kP2G, [&] { return nclr_host::launch_p2g(a); }));
+        RC_TRY(timed(kGrid, [&] { return nclr_host::launch_grid(a); }));
+        RC_TRY(timed(kG2PAdj, [&] { return nclr_host::launch_g2p_adjoint(a, b); }));
+        RC_TRY(timed(kGridAdj, [&] { return nclr_host::launch_grid_adjoint(a, b); }));
+        RC_TRY(timed(kP2GAdj, [&] { return nclr_host::launch_p2g_adjoint(a, b); }));
+        if (opts.timing) RC_TRY(record_stop(pe));
+        return MPM_OK;
+    }
+
+    // Adjoint of `nsteps` steps from the current state (a checkpoint): the states inside the
+    // segment are recomputed into the tape, then the adjoint steps run in reverse.
+    int backward_segment(int nsteps, const float* gout, float* gin, double* grad_E) {
+        RC_TRY(check_differentiable());
+        RC_TRY(ensure_adjoint());
+        if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
+        if (p.n == 0) return MPM_OK;
+        const size_t sb = state_bytes();
+        if (tape_states < nsteps) {
+            HIP_TRY(hipStreamSynchronize(stream));
+            if (tape) (void)hipFree(tape);
+            tape = nullptr;
+            tape_states = 0;
+            HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tape), sb * size_t(nsteps)));
+            tape_states = nsteps;
+        }
+        const int64_t steps0 = steps;
+        for (int t = 0; t < nsteps; ++t) {
+            RC_TRY(save_state(tape + sb * size_t(t)));
+            if (t + 1 < nsteps) RC_TRY(step());
+        }
+        steps = steps0;
+        const size_t gbytes = size_t(grad_channels()) * size_t(p.n) * sizeof(float);
+        HIP_TRY(hipMemcpyAsync(gtmp[0], gout, gbytes, hipMemcpyDeviceToDevice, stream));
+        int c = 0;
+        for (int t = nsteps - 1; t >= 0; --t) {
+            RC_TRY(load_state(tape + sb * size_t(t)));
+            RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E));
+            c ^= 1;
+        }
+        HIP_TRY(hipMemcpyAsync(gin, gtmp[c], gbytes, hipMemcpyDeviceToDevice, stream));
+        return MPM_OK;
+    }
+
+    int set_elasticity(int s, float E, float nu) {
+        if (!(std::isfinite(E) && std::isfinite(nu)) || nu <= -1.0f || nu >= 0.5f)
+            return fail(MPM_ERR_INVALID, "E must be finite and nu in (-1, 0.5)");
+        HIP_TRY(hipStreamSynchronize(stream));
+        auto& sp = scenes[size_t(s)];
+        set_scene_elasticity(sp, E, nu);
+        HIP_TRY(hipMemcpyAsync(d_scenes + s, &sp, sizeof(sp), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return MPM_OK;
+    }
+
+    static void set_scene_elasticity(nclr_dev::SceneParams& sp, float E, float nu) {
+        // src/nclr.h:77-78, float arithmetic
+        sp.mu0 = E / (2.0f * (1.0f + nu));
+        sp.lambda0 = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+        sp.dmu_dE = float(1.0 / (2.0 * (1.0 + double(nu))));
+        sp.dlambda_dE = float(double(nu) / ((1.0 + double(nu)) * (1.0 - 2.0 * double(nu))));
+    }
+
     void release() {
         if (opts.timing) (void)resolve_events();
+        drop_graphs();
         for (auto e : pool) (void)hipEventDestroy(e);
         pool.clear();
-        void* bufs[] = {state[0], state[1], pid[0], pid[1], keys, perm, keys_tmp, perm_tmp, hist, tile_start,
-                        tile_end, item_offset, slots, gv, stamps, status, active, active_sum};
+        void* bufs[] = {state[0], state[1], pid[0],  pid[1],   keys,    perm,      keys_tmp, perm_tmp,
+                        hist,     tile_start, tile_end, item_offset, slots, gv, stamps, status,
+                        active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, gtmp[0],
+                        gtmp[1],  tape};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
+        slots_b = gbar = nullptr;
+        epart = gtmp[0] = gtmp[1] = nullptr;
+        tape = nullptr;
+        tape_states = 0;
         state[0] = state[1] = nullptr;
         pid[0] = pid[1] = nullptr;
         keys = perm = keys_tmp = perm_tmp = hist = tile_start = tile_end = item_offset = nullptr;
         slots = gv = nullptr;
-        stamps = status = active = nullptr;
+        stamps = status = active = stamp = nullptr;
         active_sum = nullptr;
+        d_scenes = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
 };
 
-extern "C" {
+namespace {
 
-void mpm_default_options(mpm_options* opts) {
-    if (!opts) return;
-    std::memset(opts, 0, sizeof(*opts));
-}
-
-int mpm_create(int dim, int material, int res, float dt, float E, float nu, float gravity, int64_t n, const float* x,
-               const float* v, const float* F, const float* C, const float* Jp, const float* mass,
-               const float* volume, const mpm_options* opts, mpm_handle* out) {
+int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const float* v, const float* F,
+                const float* C, const float* Jp, const float* mass, const float* volume, const mpm_options* opts,
+                mpm_handle* out) {
     if (!out) return fail(MPM_ERR_INVALID, "out handle pointer is NULL");
     *out = nullptr;
     if (dim != 2 && dim != 3) return fail(MPM_ERR_INVALID, "dim must be 2 or 3");
-    if (material < MPM_SNOW || material > MPM_LIQUID) return fail(MPM_ERR_INVALID, "unknown material model");
-    if (res < 3 || res > (dim == 3 ? 512 : 8192)) return fail(MPM_ERR_INVALID, "res out of range");
-    if (n < 0 || n > (int64_t(1) << 31) - 64) return fail(MPM_ERR_INVALID, "particle count out of range");
+    if (nscenes < 1 || !sc) return fail(MPM_ERR_INVALID, "at least one scene is required");
+    int64_t n = 0;
+    int common = sc[0].material;
+    for (int s = 0; s < nscenes; ++s) {
+        const mpm_scene& d = sc[s];
+        if (d.material < MPM_SNOW || d.material > MPM_LIQUID) return fail(MPM_ERR_INVALID, "unknown material model");
+        if (d.res < 3 || d.res > (dim == 3 ? 512 : 8192)) return fail(MPM_ERR_INVALID, "res out of range");
+        if (d.n < 0) return fail(MPM_ERR_INVALID, "negative particle count");
+        if (!(d.dt > 0.0f) || !std::isfinite(d.dt)) return fail(MPM_ERR_INVALID, "dt must be positive and finite");
+        if (d.material != common) common = nclr_dev::kMixed;
+        n += d.n;
+    }
+    if (n > (int64_t(1) << 31) - 64) return fail(MPM_ERR_INVALID, "particle count out of range");
     if (n > 0 && !x) return fail(MPM_ERR_INVALID, "x must not be NULL");
-    if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(MPM_ERR_INVALID, "dt must be positive and finite");
 
     auto* h = new nclr_mpm_sim();
     if (opts)
@@ -279,30 +558,47 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
     else
         mpm_default_options(&h->opts);
     h->dim = dim;
-    h->material = material;
-    h->res = res;
+    h->material = common;
     auto& p = h->p;
-    p.res = res;
     const int lg = nclr_host::tile_log(dim);
-    p.nt = ((res - 2) >> lg) + 1;  // stencil bases lie in [0, res-2]
-    p.ntn = (res >> lg) + 1;       // nodes lie in [0, res]
-    uint64_t ntiles = 1;
-    h->node_tiles = 1;
-    for (int a = 0; a < dim; ++a) {
-        ntiles *= uint64_t(p.nt);
-        h->node_tiles *= p.ntn;
+    uint64_t tiles = 0, node_tiles = 0;
+    int64_t base = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        const mpm_scene& d = sc[s];
+        nclr_dev::SceneParams sp{};
+        sp.res = d.res;
+        sp.nt = ((d.res - 2) >> lg) + 1;  // stencil bases lie in [0, res-2]
+        sp.ntn = (d.res >> lg) + 1;       // nodes lie in [0, res]
+        sp.material = d.material;
+        sp.identity_mode = h->opts.identity_mode;
+        sp.tile_base = uint32_t(tiles);
+        sp.node_tile_base = uint32_t(node_tiles);
+        sp.dt = d.dt;
+        sp.dx = float(1.0 / d.res);
+        sp.inv_dx = 1.0f / sp.dx;
+        sp.gravity = d.gravity;
+        nclr_mpm_sim::set_scene_elasticity(sp, d.E, d.nu);
+        sp.allowed = float(double(sp.dx) * 0.9 / double(d.dt));
+        sp.particle_base = base;
+        sp.n = d.n;
+        uint64_t t = 1, nt = 1;
+        for (int a = 0; a < dim; ++a) {
+            t *= uint64_t(sp.nt);
+            nt *= uint64_t(sp.ntn);
+        }
+        tiles += t;
+        node_tiles += nt;
+        base += d.n;
+        h->scenes.push_back(sp);
     }
-    p.ntiles = uint32_t(ntiles);
-    p.identity_mode = h->opts.identity_mode;
-    p.stamp = 0;
-    p.dt = dt;
-    p.dx = float(1.0 / res);
-    p.inv_dx = 1.0f / p.dx;
-    p.gravity = gravity;
-    // src/nclr.h:77-78, float arithmetic
-    p.mu0 = E / (2.0f * (1.0f + nu));
-    p.lambda0 = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
-    p.allowed = float(double(p.dx) * 0.9 / double(dt));
+    if (tiles >= (uint64_t(1) << 31) || node_tiles >= (uint64_t(1) << 31)) {
+        delete h;
+        return fail(MPM_ERR_INVALID, "too many tiles");
+    }
+    p.nscenes = nscenes;
+    p.ntiles = uint32_t(tiles);
+    p.node_tiles = uint32_t(node_tiles);
+    p.chunk = 512;
     p.n = n;
     p.stride = std::max<int64_t>(64, (n + 63) / 64 * 64);
 
@@ -311,13 +607,20 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
         delete h;
         return rc;
     };
+    {
+        const int rc = h->set_tuning(h->opts.p2g_chunk, h->opts.g2p_threads);
+        if (rc != MPM_OK) {
+            delete h;
+            return rc;
+        }
+    }
     if (hipSetDevice(h->opts.device) != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "hipSetDevice failed"));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipStreamCreate failed"));
     const size_t state_floats = size_t(h->nf()) * size_t(p.stride);
-    const size_t nbins = size_t(ntiles) + 1;
-    const size_t gv_nodes = size_t(h->node_tiles) * nclr_host::tile_nodes(dim);
-    const size_t slot_nodes = size_t(nclr_host::max_items(dim, n, uint32_t(ntiles))) * nclr_host::region_nodes(dim);
+    const size_t nbins = size_t(tiles) + 1;
+    const size_t gv_nodes = size_t(node_tiles) * nclr_host::tile_nodes(dim);
+    const size_t slot_nodes = size_t(nclr_host::max_items(n, uint32_t(tiles))) * nclr_host::region_nodes(dim);
     const size_t nn = size_t(std::max<int64_t>(n, 1));
     if (dmalloc(&h->state[0], state_floats) != hipSuccess || dmalloc(&h->state[1], state_floats) != hipSuccess ||
         dmalloc(&h->pid[0], size_t(p.stride)) != hipSuccess || dmalloc(&h->pid[1], size_t(p.stride)) != hipSuccess ||
@@ -326,15 +629,21 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
         dmalloc(&h->hist, size_t(256) * size_t(nclr_host::sort_tiles(int64_t(nn))) + 256) != hipSuccess ||
         dmalloc(&h->tile_start, nbins) != hipSuccess || dmalloc(&h->tile_end, nbins) != hipSuccess ||
         dmalloc(&h->item_offset, nbins) != hipSuccess || dmalloc(&h->slots, slot_nodes) != hipSuccess ||
-        dmalloc(&h->gv, gv_nodes) != hipSuccess || dmalloc(&h->stamps, size_t(h->node_tiles)) != hipSuccess ||
-        dmalloc(&h->status, 1) != hipSuccess || dmalloc(&h->active, size_t(h->node_tiles)) != hipSuccess ||
-        dmalloc(&h->active_sum, 1) != hipSuccess)
+        dmalloc(&h->gv, gv_nodes) != hipSuccess || dmalloc(&h->stamps, size_t(node_tiles)) != hipSuccess ||
+        dmalloc(&h->status, 1) != hipSuccess || dmalloc(&h->active, size_t(node_tiles)) != hipSuccess ||
+        dmalloc(&h->active_sum, 1) != hipSuccess || dmalloc(&h->stamp, 1) != hipSuccess ||
+        dmalloc(&h->d_scenes, size_t(nscenes)) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
-    if (nclr_host::launch_zero_f4(h->gv, int64_t(gv_nodes), h->stream) != hipSuccess ||
+    p.scenes = h->d_scenes;
+    p.stamp = h->stamp;
+    if (hipMemcpyAsync(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(nclr_dev::SceneParams),
+                       hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        nclr_host::launch_zero_f4(h->gv, int64_t(gv_nodes), h->stream) != hipSuccess ||
         nclr_host::launch_zero_f4(h->slots, int64_t(slot_nodes), h->stream) != hipSuccess ||
-        nclr_host::launch_fill_u32(h->stamps, h->node_tiles, 0u, h->stream) != hipSuccess ||
+        nclr_host::launch_fill_u32(h->stamps, int64_t(node_tiles), 0u, h->stream) != hipSuccess ||
+        nclr_host::launch_fill_u32(h->stamp, 1, 0u, h->stream) != hipSuccess ||
         nclr_host::launch_fill_u32(h->status, 1, 0u, h->stream) != hipSuccess ||
-        nclr_host::launch_fill_u32(h->active, h->node_tiles, 0u, h->stream) != hipSuccess ||
+        nclr_host::launch_fill_u32(h->active, int64_t(node_tiles), 0u, h->stream) != hipSuccess ||
         nclr_host::launch_fill_u32(h->pid[1], p.stride, 0u, h->stream) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "device initialisation failed"));
 
@@ -350,24 +659,48 @@ int mpm_create(int dim, int material, int res, float dt, float E, float nu, floa
         host[size_t(fJ + 1) * p.stride + i] = 1.0f;
         host[size_t(fJ + 2) * p.stride + i] = 1.0f;
     }
-    const int rc = h->upload(x, v, F, C, Jp, mass, volume, host);
+    int rc = h->upload(x, v, F, C, Jp, mass, volume, host);
     if (rc != MPM_OK) return cleanup(rc);
     if (hipMemcpyAsync(h->state[1], h->state[0], state_floats * sizeof(float), hipMemcpyDeviceToDevice, h->stream) !=
             hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "device initialisation failed"));
+    if (h->opts.autotune && n > 0) {
+        rc = h->autotune(3);
+        if (rc != MPM_OK) return cleanup(rc);
+    }
     *out = h;
     return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mpm_default_options(mpm_options* opts) {
+    if (!opts) return;
+    std::memset(opts, 0, sizeof(*opts));
+}
+
+int mpm_create(int dim, int material, int res, float dt, float E, float nu, float gravity, int64_t n, const float* x,
+               const float* v, const float* F, const float* C, const float* Jp, const float* mass,
+               const float* volume, const mpm_options* opts, mpm_handle* out) {
+    if (n < 0) return fail(MPM_ERR_INVALID, "particle count out of range");
+    const mpm_scene sc{material, res, dt, E, nu, gravity, n};
+    return create_impl(dim, 1, &sc, x, v, F, C, Jp, mass, volume, opts, out);
+}
+
+int mpm_create_batch(int dim, int nscenes, const mpm_scene* scenes, const float* x, const float* v, const float* F,
+                     const float* C, const float* Jp, const float* mass, const float* volume,
+                     const mpm_options* opts, mpm_handle* out) {
+    return create_impl(dim, nscenes, scenes, x, v, F, C, Jp, mass, volume, opts, out);
 }
 
 int mpm_advance(mpm_handle h, int nsteps) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
     if (nsteps < 0) return fail(MPM_ERR_INVALID, "nsteps must be >= 0");
     HIP_TRY(hipSetDevice(h->opts.device));
-    for (int s = 0; s < nsteps; ++s) {
-        const int rc = h->step();
-        if (rc != MPM_OK) return rc;
-    }
+    for (int s = 0; s < nsteps; ++s) RC_TRY(h->step());
     return MPM_OK;
 }
 
@@ -379,12 +712,10 @@ int mpm_synchronize(mpm_handle h) {
 int mpm_get_particles(mpm_handle h, float* x, float* v, float* F, float* C, float* Jp, float* mass,
                       float* volume) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
-    int rc = h->sync();
-    if (rc != MPM_OK) return rc;
+    RC_TRY(h->sync());
     std::vector<float> host;
     std::vector<uint32_t> ids;
-    rc = h->download(host, ids);
-    if (rc != MPM_OK) return rc;
+    RC_TRY(h->download(host, ids));
     h->unpermute(host, ids);
     const int D = h->dim;
     const int64_t n = h->p.n, stride = h->p.stride;
@@ -408,21 +739,20 @@ int mpm_get_particles(mpm_handle h, float* x, float* v, float* F, float* C, floa
 int mpm_set_particles(mpm_handle h, const float* x, const float* v, const float* F, const float* C,
                       const float* Jp, const float* mass, const float* volume) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
-    int rc = h->sync();
-    if (rc != MPM_OK) return rc;
+    RC_TRY(h->sync());
     std::vector<float> host;
     std::vector<uint32_t> ids;
-    rc = h->download(host, ids);
-    if (rc != MPM_OK) return rc;
+    RC_TRY(h->download(host, ids));
     h->unpermute(host, ids);
     return h->upload(x, v, F, C, Jp, mass, volume, host);
 }
 
-int mpm_get_grid(mpm_handle h, float* velocity, float* mass) {
+int mpm_get_grid_scene(mpm_handle h, int scene, float* velocity, float* mass) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
-    int rc = h->sync();
-    if (rc != MPM_OK) return rc;
-    const int64_t total = h->num_nodes();
+    if (scene < 0 || scene >= int(h->scenes.size())) return fail(MPM_ERR_INVALID, "scene index out of range");
+    RC_TRY(h->sync());
+    const auto& sp = h->scenes[size_t(scene)];
+    const int64_t total = h->scene_nodes(scene);
     float* dvel = nullptr;
     float* dmass = nullptr;
     HIP_TRY(dmalloc(&dvel, size_t(total) * h->dim));
@@ -430,7 +760,8 @@ int mpm_get_grid(mpm_handle h, float* velocity, float* mass) {
         (void)hipFree(dvel);
         return fail(MPM_ERR_HIP, "hipMalloc failed");
     }
-    hipError_t e = nclr_host::launch_export_grid(h->args(), dvel, dmass, total);
+    hipError_t e = nclr_host::launch_export_grid(h->args(h->cur), scene, sp.res, sp.node_tile_base, sp.ntn, dvel,
+                                                 dmass, total);
     if (e == hipSuccess && velocity)
         e = hipMemcpyAsync(velocity, dvel, size_t(total) * h->dim * sizeof(float), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess && mass)
@@ -442,18 +773,32 @@ int mpm_get_grid(mpm_handle h, float* velocity, float* mass) {
     return MPM_OK;
 }
 
-int mpm_get_params(mpm_handle h, float* mu_0, float* lambda_0, float* dx, float* inv_dx) {
+int mpm_get_grid(mpm_handle h, float* velocity, float* mass) { return mpm_get_grid_scene(h, 0, velocity, mass); }
+
+int mpm_get_scene_params(mpm_handle h, int scene, float* mu_0, float* lambda_0, float* dx, float* inv_dx) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
-    if (mu_0) *mu_0 = h->p.mu0;
-    if (lambda_0) *lambda_0 = h->p.lambda0;
-    if (dx) *dx = h->p.dx;
-    if (inv_dx) *inv_dx = h->p.inv_dx;
+    if (scene < 0 || scene >= int(h->scenes.size())) return fail(MPM_ERR_INVALID, "scene index out of range");
+    const auto& sp = h->scenes[size_t(scene)];
+    if (mu_0) *mu_0 = sp.mu0;
+    if (lambda_0) *lambda_0 = sp.lambda0;
+    if (dx) *dx = sp.dx;
+    if (inv_dx) *inv_dx = sp.inv_dx;
     return MPM_OK;
 }
 
+int mpm_get_params(mpm_handle h, float* mu_0, float* lambda_0, float* dx, float* inv_dx) {
+    return mpm_get_scene_params(h, 0, mu_0, lambda_0, dx, inv_dx);
+}
+
 int64_t mpm_num_particles(mpm_handle h) { return h ? h->p.n : -1; }
-int64_t mpm_num_nodes(mpm_handle h) { return h ? h->num_nodes() : -1; }
+int64_t mpm_num_nodes(mpm_handle h) {
+    if (!h) return -1;
+    int64_t t = 0;
+    for (size_t s = 0; s < h->scenes.size(); ++s) t += h->scene_nodes(int(s));
+    return t;
+}
 int64_t mpm_steps_taken(mpm_handle h) { return h ? h->steps : -1; }
+int mpm_num_scenes(mpm_handle h) { return h ? int(h->scenes.size()) : -1; }
 
 int mpm_kernel_time(mpm_handle h, const char* name, double* avg_ms, int64_t* launches) {
     if (!h || !name) return fail(MPM_ERR_INVALID, "NULL argument");
@@ -462,8 +807,7 @@ int mpm_kernel_time(mpm_handle h, const char* name, double* avg_ms, int64_t* lau
         if (std::strcmp(name, kKernelNames[i]) == 0) k = i;
     if (k < 0) return fail(MPM_ERR_INVALID, std::string("unknown kernel name: ") + name);
     HIP_TRY(hipStreamSynchronize(h->stream));
-    const int rc = h->resolve_events();
-    if (rc != MPM_OK) return rc;
+    RC_TRY(h->resolve_events());
     const auto& t = h->timers[k];
     if (avg_ms) *avg_ms = t.launches ? t.total_ms / double(t.launches) : 0.0;
     if (launches) *launches = t.launches;
@@ -473,8 +817,7 @@ int mpm_kernel_time(mpm_handle h, const char* name, double* avg_ms, int64_t* lau
 int mpm_reset_kernel_times(mpm_handle h) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
     HIP_TRY(hipStreamSynchronize(h->stream));
-    const int rc = h->resolve_events();
-    if (rc != MPM_OK) return rc;
+    RC_TRY(h->resolve_events());
     for (auto& t : h->timers) t = KernelTimer{};
     return MPM_OK;
 }
@@ -482,27 +825,51 @@ int mpm_reset_kernel_times(mpm_handle h) {
 int mpm_set_timing(mpm_handle h, int enable) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
     HIP_TRY(hipStreamSynchronize(h->stream));
-    const int rc = h->resolve_events();
-    if (rc != MPM_OK) return rc;
+    RC_TRY(h->resolve_events());
     h->opts.timing = enable ? 1 : 0;
     return MPM_OK;
 }
 
-int mpm_active_nodes(mpm_handle h, int64_t* count) {
+int mpm_set_tuning(mpm_handle h, int p2g_chunk, int g2p_threads) {
+    if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
+    return h->set_tuning(p2g_chunk, g2p_threads);
+}
+
+int mpm_get_tuning(mpm_handle h, int* p2g_chunk, int* g2p_threads) {
+    if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
+    if (p2g_chunk) *p2g_chunk = h->p.chunk;
+    if (g2p_threads) *g2p_threads = h->g2p_threads;
+    return MPM_OK;
+}
+
+int mpm_autotune(mpm_handle h, int steps_per_trial) {
+    if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
+    if (h->p.n == 0) return MPM_OK;
+    return h->autotune(steps_per_trial);
+}
+
+int mpm_active_nodes_scene(mpm_handle h, int scene, int64_t* count) {
     if (!h || !count) return fail(MPM_ERR_INVALID, "NULL argument");
-    int rc = h->sync();
-    if (rc != MPM_OK) return rc;
-    if (h->p.stamp == 0) {
+    if (scene < -1 || scene >= int(h->scenes.size())) return fail(MPM_ERR_INVALID, "scene index out of range");
+    RC_TRY(h->sync());
+    if (h->steps == 0) {
         *count = 0;
         return MPM_OK;
     }
+    uint32_t first = 0, tiles = h->p.node_tiles;
+    if (scene >= 0) {
+        first = h->scenes[size_t(scene)].node_tile_base;
+        tiles = (size_t(scene) + 1 < h->scenes.size() ? h->scenes[size_t(scene) + 1].node_tile_base : h->p.node_tiles) - first;
+    }
     unsigned long long c = 0;
-    HIP_TRY(nclr_host::launch_active_sum(h->args(), h->active_sum));
+    HIP_TRY(nclr_host::launch_active_sum(h->args(h->cur), first, tiles, h->active_sum));
     HIP_TRY(hipMemcpyAsync(&c, h->active_sum, sizeof(c), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     *count = int64_t(c);
     return MPM_OK;
 }
+
+int mpm_active_nodes(mpm_handle h, int64_t* count) { return mpm_active_nodes_scene(h, -1, count); }
 
 int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int device) {
     if (n < 0 || (n > 0 && (!keys || !perm))) return fail(MPM_ERR_INVALID, "bad arguments");
@@ -530,6 +897,112 @@ int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int
     return MPM_OK;
 }
 
+int mpm_svd(int dim, int64_t n, const float* A, float* U, float* sigma, float* V, int exact, int device) {
+    if (dim != 2 && dim != 3) return fail(MPM_ERR_INVALID, "dim must be 2 or 3");
+    if (n < 0 || (n > 0 && (!A || !U || !sigma || !V))) return fail(MPM_ERR_INVALID, "bad arguments");
+    if (n == 0) return MPM_OK;
+    HIP_TRY(hipSetDevice(device));
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const size_t mats = size_t(n) * dim * dim, vecs = size_t(n) * dim;
+    float *dA = nullptr, *dU = nullptr, *dS = nullptr, *dV = nullptr;
+    hipError_t e = dmalloc(&dA, mats);
+    if (e == hipSuccess) e = dmalloc(&dU, mats);
+    if (e == hipSuccess) e = dmalloc(&dS, vecs);
+    if (e == hipSuccess) e = dmalloc(&dV, mats);
+    if (e == hipSuccess) e = hipMemcpyAsync(dA, A, mats * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = nclr_host::launch_svd(dim, exact != 0, dA, dU, dS, dV, n, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(U, dU, mats * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(sigma, dS, vecs * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(V, dV, mats * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    void* bufs[] = {dA, dU, dS, dV};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("mpm_svd: ") + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+int mpm_grad_channels(int dim) { return (dim == 2 || dim == 3) ? 2 * dim + 2 * dim * dim : -1; }
+
+int64_t mpm_state_bytes(mpm_handle h) { return h ? int64_t(h->state_bytes()) : -1; }
+
+int mpm_save_state(mpm_handle h, void* dst) {
+    if (!h || !dst) return fail(MPM_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    return h->save_state(dst);
+}
+
+int mpm_load_state(mpm_handle h, const void* src) {
+    if (!h || !src) return fail(MPM_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    return h->load_state(src);
+}
+
+int mpm_get_state_device(mpm_handle h, float* dst) {
+    if (!h || (!dst && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    HIP_TRY(nclr_host::launch_permute_channels(h->pid[h->cur], h->state[h->cur], h->p.stride, dst, h->p.n,
+                                               h->grad_channels(), h->p.n, true, h->stream));
+    return MPM_OK;
+}
+
+int mpm_set_state_device(mpm_handle h, const float* src) {
+    if (!h || (!src && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    const int64_t n = h->p.n, stride = h->p.stride;
+    if (n == 0) return MPM_OK;
+    const int o = h->cur ^ 1, ng = h->grad_channels();
+    // Jp, mass and volume into particle order, then x, v, F, C from the caller; slots = particles
+    HIP_TRY(nclr_host::launch_permute_channels(h->pid[h->cur], h->state[h->cur] + int64_t(ng) * stride, stride,
+                                               h->state[o] + int64_t(ng) * stride, stride, h->nf() - ng, n, true,
+                                               h->stream));
+    HIP_TRY(hipMemcpy2DAsync(h->state[o], size_t(stride) * sizeof(float), src, size_t(n) * sizeof(float),
+                             size_t(n) * sizeof(float), size_t(ng), hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(nclr_host::launch_iota_u32(h->pid[o], stride, h->stream));
+    h->cur = o;
+    return MPM_OK;
+}
+
+int mpm_grad_from_particle_order(mpm_handle h, const float* src, float* dst) {
+    if (!h || ((!src || !dst) && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
+    if (src == dst && h->p.n > 0) return fail(MPM_ERR_INVALID, "src and dst must not alias");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    HIP_TRY(nclr_host::launch_permute_channels(h->pid[h->cur], src, h->p.n, dst, h->p.n, h->grad_channels(), h->p.n,
+                                               false, h->stream));
+    return MPM_OK;
+}
+
+int mpm_grad_to_particle_order(mpm_handle h, const float* src, float* dst) {
+    if (!h || ((!src || !dst) && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
+    if (src == dst && h->p.n > 0) return fail(MPM_ERR_INVALID, "src and dst must not alias");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    HIP_TRY(nclr_host::launch_permute_channels(h->pid[h->cur], src, h->p.n, dst, h->p.n, h->grad_channels(), h->p.n,
+                                               true, h->stream));
+    return MPM_OK;
+}
+
+int mpm_backward_step(mpm_handle h, const float* grad_out, float* grad_in, double* grad_E) {
+    if (!h || ((!grad_out || !grad_in) && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
+    if (grad_out == grad_in && h->p.n > 0) return fail(MPM_ERR_INVALID, "grad_out and grad_in must not alias");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    return h->backward_step(grad_out, grad_in, grad_E);
+}
+
+int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float* grad_in, double* grad_E) {
+    if (!h || ((!grad_out || !grad_in) && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    return h->backward_segment(nsteps, grad_out, grad_in, grad_E);
+}
+
+int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu) {
+    if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
+    if (scene < 0 || scene >= int(h->scenes.size())) return fail(MPM_ERR_INVALID, "scene index out of range");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    return h->set_elasticity(scene, E, nu);
+}
+
 int mpm_destroy(mpm_handle h) {
     if (!h) return MPM_OK;
     (void)hipSetDevice(h->opts.device);
@@ -542,8 +1015,8 @@ int mpm_destroy(mpm_handle h) {
 const char* mpm_last_error(void) { return g_err.c_str(); }
 
 const char* mpm_version(void) {
-    return "nclr_mpm 0.2 gfx950 (fp32, no fp contraction, IEEE div/sqrt, tile-sorted SoA particles, "
-           "LDS tile P2G)";
+    return "nclr_mpm 0.3 gfx950 (fp32, no fp contraction, IEEE div/sqrt, tile-sorted SoA particles, "
+           "fixed-point LDS tile P2G, batched scenes, hipGraph steps)";
 }
 
 }  // extern "C"

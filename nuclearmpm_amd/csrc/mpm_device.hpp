@@ -1,0 +1,170 @@
+// Device helpers shared by the step kernels (mpm_step.hip) and their adjoints (mpm_adjoint.hip):
+// the particle field layout, scene lookup, the interpolation stencil, hardening, the fused
+// affine matrix of P2G and the tile / node-tile index maps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mpm_internal.hpp"
+#include "mpm_math.hpp"
+
+namespace nclr_dev {
+
+// Particle fields: x[D] v[D] F[DxD] C[DxD] Jp mass volume, field f of slot i at st[f*stride + i].
+// The first NG = 2D + 2D^2 fields are also the channels of a particle-state cotangent.
+template <int D>
+struct Layout {
+    static constexpr int X = 0;
+    static constexpr int V = D;
+    static constexpr int F = 2 * D;
+    static constexpr int C = 2 * D + D * D;
+    static constexpr int JP = 2 * D + 2 * D * D;
+    static constexpr int MASS = JP + 1;
+    static constexpr int VOL = JP + 2;
+    static constexpr int NF = JP + 3;
+    static constexpr int NG = JP;
+};
+
+// Largest scene index s with key(s) <= v (scenes ordered by every base).
+template <typename KeyFn>
+NCLR_DI int scene_search(int nscenes, int64_t v, KeyFn key) {
+    int lo = 0, hi = nscenes;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (key(mid) <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+NCLR_DI int scene_of_particle(const StepParams& p, uint32_t pid) {
+    return scene_search(p.nscenes, pid, [&](int s) { return p.scenes[s].particle_base; });
+}
+NCLR_DI int scene_of_tile(const StepParams& p, uint32_t tile) {
+    return scene_search(p.nscenes, tile, [&](int s) { return int64_t(p.scenes[s].tile_base); });
+}
+NCLR_DI int scene_of_node_tile(const StepParams& p, uint32_t nt) {
+    return scene_search(p.nscenes, nt, [&](int s) { return int64_t(p.scenes[s].node_tile_base); });
+}
+
+// base = trunc(x*inv_dx - 0.5), fx = x*inv_dx - base (src/nclr.h:115,121), with the stencil
+// nodes base..base+2 required to lie in [0, res] on every axis.
+template <int D>
+NCLR_DI bool stencil(const float (&x)[D], const SceneParams& sp, int (&base)[D], float (&fx)[D], float (&w)[D][3]) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const float s = x[a] * sp.inv_dx;
+        const float b = s - 0.5f;
+        ok = ok && (b > -1.0f) && (b < float(sp.res - 1));
+        base[a] = ok ? int(b) : 0;
+        fx[a] = s - float(base[a]);
+        bspline_weights(fx[a], w[a]);
+    }
+    return ok;
+}
+
+NCLR_DI float ldf(const float* __restrict__ st, int64_t stride, int f, int64_t i) { return st[int64_t(f) * stride + i]; }
+
+// hardening (src/nclr.h:351-372): snow e = exp(10 (1 - Jp)) evaluated in double
+NCLR_DI float hardening(int material, float Jp) {
+    if (material == MPM_SNOW) return float(exp(10.0 * (1.0 - double(Jp))));
+    if (material == MPM_JELLY) return 0.3f;
+    return 1.0f;
+}
+
+// first_piola_kirchoff_stress (src/nclr.h:313-337): -(dt V0)(4 inv_dx^2 PF) + m C with
+// PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry.
+template <int D, bool EXACT>
+NCLR_DI M<D> fused_affine(int material, const M<D>& F, const M<D>& C, float Jp, float mass, float vol,
+                          const SceneParams& sp) {
+    const float e = hardening(material, Jp);
+    const float mu = sp.mu0 * e;
+    const float lambda = sp.lambda0 * e;
+    const float J = det(F);
+    const M<D> R = polar_rotation_mode<EXACT>(F);
+    const float Dinv = 4.0f * sp.inv_dx * sp.inv_dx;
+    const float two_mu = 2.0f * mu;
+    M<D> FmR;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) FmR.a[a][b] = two_mu * (F.a[a][b] - R.a[a][b]);
+    const M<D> PF = matmul_bt<D>(FmR, F);
+    const float lam_term = lambda * (J - 1.0f) * J;
+    const float neg_dtv = -(sp.dt * vol);
+    M<D> A;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) A.a[a][b] = neg_dtv * (Dinv * (PF.a[a][b] + lam_term)) + mass * C.a[a][b];
+    return A;
+}
+
+template <int D>
+NCLR_DI void offsets(int o, int (&off)[D]) {
+    if constexpr (D == 2) {
+        off[0] = o / 3;
+        off[1] = o % 3;
+    } else {
+        off[0] = o / 9;
+        off[1] = (o / 3) % 3;
+        off[2] = o % 3;
+    }
+}
+
+// Index of node n of the scene whose node tiles start at node_tile_base in the updated grid.
+template <int D>
+NCLR_DI int64_t gv_index(const int (&n)[D], int ntn, uint32_t node_tile_base) {
+    using TL = Tile<D>;
+    int64_t t = 0;
+    int local = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        t = t * ntn + (n[a] >> TL::LOG);
+        local = local * TL::S + (n[a] & (TL::S - 1));
+    }
+    return (int64_t(node_tile_base) + t) * TL::NODES + local;
+}
+
+template <int D>
+NCLR_DI uint32_t tile_key(const int (&base)[D], int nt) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) t = t * uint32_t(nt) + uint32_t(base[a] >> Tile<D>::LOG);
+    return t;
+}
+
+template <int D>
+NCLR_DI void tile_coords(uint32_t t, int n, int (&c)[D]) {
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) {
+        c[a] = int(t % uint32_t(n));
+        t /= uint32_t(n);
+    }
+}
+
+// Work item -> (tile, chunk within the tile) by binary search over the exclusive item scan.
+NCLR_DI void item_tile(const uint32_t* __restrict__ item_offset, uint32_t ntiles, uint32_t item, uint32_t& tile,
+                       uint32_t& chunk) {
+    uint32_t lo = 0, hi = ntiles;  // item_offset[lo] <= item < item_offset[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (item_offset[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    tile = lo;
+    chunk = item - item_offset[lo];
+}
+
+// Per-channel fixed-point scale 2^e for a workgroup whose summed |contribution| bound is `total`:
+// total * 2^e <= 2^61, so the 64-bit integer sum of the quantised values cannot overflow.
+NCLR_DI void fixed_point_scale(float total, float& scale, float& inv_scale) {
+    int ex = 0;
+    (void)frexpf(total > 0.0f && total < 3.0e38f ? total : 1.0f, &ex);  // total < 2^ex
+    const int e = min(127, 61 - ex);                                      // in [-68, 127]
+    scale = ldexpf(1.0f, e);
+    inv_scale = ldexpf(1.0f, -e);
+}
+
+}  // namespace nclr_dev

@@ -10,6 +10,7 @@
 namespace nclr_dev {
 
 constexpr uint32_t kStatusOutOfRange = 1u;
+constexpr int kMixed = 3;  // material template value: per-scene material chosen at run time
 
 // Particle tiles: S^D cells (S = 16 in 2D, 8 in 3D).  A tile's P2G region covers the nodes its
 // particles can touch, (S+2)^D nodes from the tile origin.  Node tiles hold S^D nodes each.
@@ -22,7 +23,6 @@ struct Tile<2> {
     static constexpr int R = 18;           // region side
     static constexpr int REGION = 18 * 18;  // region nodes
     static constexpr int NODES = 16 * 16;   // nodes per node tile
-    static constexpr int CHUNK = 1024;      // particles per P2G work item
 };
 template <>
 struct Tile<3> {
@@ -31,22 +31,38 @@ struct Tile<3> {
     static constexpr int R = 10;
     static constexpr int REGION = 10 * 10 * 10;
     static constexpr int NODES = 8 * 8 * 8;
-    static constexpr int CHUNK = 512;
 };
 
-// Scalars of one simulation, passed by value to every kernel.
-struct StepParams {
-    int res;            // grid resolution; nodes per axis = res + 1
-    int nt;             // particle tiles per axis
-    int ntn;            // node tiles per axis
-    int identity_mode;  // 0 reference diag<dim>(1), 1 physical identity
-    uint32_t stamp;     // step stamp written into the node tiles the step touched
-    uint32_t ntiles;    // nt^D; also the key of particles outside the grid
+// Smallest P2G work item (particles per workgroup); the slot buffer is sized for it.
+constexpr int kMinChunk = 256;
+
+// One scene (an independent simulation) of a batch.
+struct SceneParams {
+    int res;             // grid resolution; nodes per axis = res + 1
+    int nt;              // particle tiles per axis
+    int ntn;             // node tiles per axis
+    int material;        // MPM_SNOW / MPM_JELLY / MPM_LIQUID
+    int identity_mode;   // 0 reference diag<dim>(1), 1 physical identity
+    uint32_t tile_base;  // first global particle tile
+    uint32_t node_tile_base;
     float dt, dx, inv_dx, gravity;
     float mu0, lambda0;
-    float allowed;      // grid velocity clamp, float(double(dx) * 0.9 / double(dt))
-    int64_t n;          // particles
-    int64_t stride;     // floats between consecutive fields of the SoA state
+    float allowed;       // grid velocity clamp, float(double(dx) * 0.9 / double(dt))
+    float dmu_dE, dlambda_dE;  // d mu0 / dE = 1 / (2 (1 + nu)), d lambda0 / dE = nu / ((1 + nu)(1 - 2 nu))
+    int64_t particle_base;  // first particle (original index)
+    int64_t n;
+};
+
+// Launch-wide values, passed by value to every kernel.
+struct StepParams {
+    int nscenes;
+    uint32_t ntiles;      // particle tiles of all scenes; also the key of particles off their grid
+    uint32_t node_tiles;  // node tiles of all scenes
+    int chunk;            // P2G particles per work item (256, 512 or 1024)
+    int64_t n;            // particles of all scenes
+    int64_t stride;       // floats between consecutive fields of the SoA state
+    const SceneParams* scenes;
+    const uint32_t* stamp;  // device word: stamp of the current step (incremented by the key kernel)
 };
 
 }  // namespace nclr_dev
@@ -55,8 +71,11 @@ namespace nclr_host {
 
 struct LaunchArgs {
     int dim;
-    int material;
+    int material;  // common material of all scenes, or nclr_dev::kMixed
+    int g2p_threads;
+    int exact_svd;  // 1: 3D SVDs in nclr_svd's operation order (bitwise the oracle), 0: fast path
     nclr_dev::StepParams p;
+    uint32_t* stamp;           // device stamp word (same as p.stamp)
     const float* state_in;     // SoA state, current physical order
     float* state_out;          // SoA state written by G2P in sorted order
     const uint32_t* pid_in;    // original particle index of each slot
@@ -77,19 +96,39 @@ struct LaunchArgs {
     hipStream_t stream;
 };
 
+// Buffers of one adjoint step (mpm_adjoint.hip).  Cotangents are structure-of-arrays with
+// Layout<D>::NG channels (x, v, F, C) of `gstride` floats each.
+struct AdjointArgs {
+    const float* grad_out;  // cotangent of the state after the step, in sorted order (its slot order)
+    float* grad_in;         // cotangent of the state before the step, in the current slot order
+    int64_t gstride;
+    float4* slots_b;        // max_items * REGION: per work item, the G2P adjoint's grid cotangent
+    float4* gbar;           // node tiles * NODES: cotangent of the pre-update grid {momentum, mass}
+    float* epart;           // n: per sorted particle, its contribution to dL/dE of its scene
+    double* grad_E;         // nscenes: accumulated dL/dE
+};
+
+hipError_t launch_g2p_adjoint(const LaunchArgs& a, const AdjointArgs& b);
+hipError_t launch_grid_adjoint(const LaunchArgs& a, const AdjointArgs& b);
+hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b);
+hipError_t launch_permute_channels(const uint32_t* pid, const float* src, int64_t src_stride, float* dst,
+                                   int64_t dst_stride, int channels, int64_t n, bool to_particle_order,
+                                   hipStream_t s);
 hipError_t launch_sort(const LaunchArgs& a);
 hipError_t launch_p2g(const LaunchArgs& a);
 hipError_t launch_grid(const LaunchArgs& a);
 hipError_t launch_g2p(const LaunchArgs& a);
-hipError_t launch_export_grid(const LaunchArgs& a, float* vel, float* mass, int64_t total);
+hipError_t launch_export_grid(const LaunchArgs& a, int scene, int res, uint32_t node_tile_base, int ntn,
+                              float* vel, float* mass, int64_t total);
+hipError_t launch_svd(int dim, bool exact, const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t s);
 hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
 hipError_t launch_zero_f4(float4* p, int64_t n, hipStream_t s);
+hipError_t launch_active_sum(const LaunchArgs& a, uint32_t first_tile, uint32_t tiles, unsigned long long* out);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* hist,
                             int64_t n, int bits, bool identity_values, hipStream_t s);
-hipError_t launch_active_sum(const LaunchArgs& a, unsigned long long* out);
 int64_t sort_tiles(int64_t n);
-int64_t max_items(int dim, int64_t n, uint32_t ntiles);
+int64_t max_items(int64_t n, uint32_t ntiles);
 int num_fields(int dim);
 int region_nodes(int dim);
 int tile_nodes(int dim);

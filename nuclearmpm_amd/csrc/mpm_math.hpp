@@ -243,6 +243,154 @@ NCLR_DI void nclr_svd(const M<D>& A, M<D>& U, float (&sig)[D], M<D>& V) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fast 3x3 SVD with nclr_svd's convention (U, V in SO(3), singular values in descending order of
+// magnitude, the sign of det A on the last one), for the default (non-bitwise) mode.
+//
+// Cyclic Jacobi eigen-decomposition of the symmetric B = A^T A (Golub & Van Loan, Matrix
+// Computations, 4th ed., Algorithm 8.5.2 with the 2x2 symmetric Schur step of Algorithm 8.5.1),
+// a fixed four sweeps (quadratic convergence reaches FP32 rounding in three; the fixed count keeps
+// every lane of a wave on the same path), then U from A V by Gram-Schmidt with u3 = u1 x u2, and
+// sigma_i = u_i^T A v_i.  Singular vectors of small singular values come from the eigenvectors of
+// B (well separated from the large ones) rather than from A v / sigma, so rank-deficient inputs
+// (the reference's default F = diag(1, 1, 0) in 3D) keep orthonormal factors.
+// Rotation, reciprocal and square-root steps use the hardware approximations (1 ulp): the
+// factors are orthonormal to FP32 rounding by construction (c = 1/sqrt(1 + t^2), s = t c).
+// ---------------------------------------------------------------------------------------------
+NCLR_DI float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+NCLR_DI float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+NCLR_DI float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <int P, int Q>
+NCLR_DI void sym_jacobi(float (&B)[3][3], M<3>& V) {
+    constexpr int R = 3 - P - Q;
+    const float bpq = B[P][Q];
+    const float bpp = B[P][P], bqq = B[Q][Q];
+    // tan of the angle that zeroes B[P][Q]; |tau| -> inf gives t -> 0
+    const float tau = (bqq - bpp) * fast_rcp(2.0f * bpq);
+    float t = copysignf(1.0f, tau) * fast_rcp(fabsf(tau) + fast_sqrt(fmaf(tau, tau, 1.0f)));
+    t = (bpq != 0.0f && t == t) ? t : 0.0f;
+    const float c = fast_rsq(fmaf(t, t, 1.0f));
+    const float s = t * c;
+    B[P][P] = fmaf(-t, bpq, bpp);
+    B[Q][Q] = fmaf(t, bpq, bqq);
+    B[P][Q] = B[Q][P] = 0.0f;
+    const float brp = B[R][P], brq = B[R][Q];
+    B[R][P] = B[P][R] = fmaf(c, brp, -s * brq);
+    B[R][Q] = B[Q][R] = fmaf(s, brp, c * brq);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float vp = V.a[k][P], vq = V.a[k][Q];
+        V.a[k][P] = fmaf(c, vp, -s * vq);
+        V.a[k][Q] = fmaf(s, vp, c * vq);
+    }
+}
+
+// Orders eigenpairs I < J by decreasing eigenvalue; a column swap is paired with a sign change so
+// that det V stays +1.
+template <int I, int J>
+NCLR_DI void sort_pair(float (&l)[3], M<3>& V) {
+    const bool sw = l[I] < l[J];
+    const float li = l[I], lj = l[J];
+    l[I] = sw ? lj : li;
+    l[J] = sw ? li : lj;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float vi = V.a[k][I], vj = V.a[k][J];
+        V.a[k][I] = sw ? vj : vi;
+        V.a[k][J] = sw ? -vi : vj;
+    }
+}
+
+NCLR_DI void fast_svd3(const M<3>& A, M<3>& U, float (&sig)[3], M<3>& V) {
+    float B[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            float s = A.a[0][i] * A.a[0][j];
+            s = fmaf(A.a[1][i], A.a[1][j], s);
+            s = fmaf(A.a[2][i], A.a[2][j], s);
+            B[i][j] = s;
+            B[j][i] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) V.a[i][j] = i == j ? 1.0f : 0.0f;
+#pragma unroll
+    for (int sweep = 0; sweep < 4; ++sweep) {
+        sym_jacobi<0, 1>(B, V);
+        sym_jacobi<0, 2>(B, V);
+        sym_jacobi<1, 2>(B, V);
+    }
+    float l[3] = {B[0][0], B[1][1], B[2][2]};
+    sort_pair<0, 1>(l, V);
+    sort_pair<0, 2>(l, V);
+    sort_pair<1, 2>(l, V);
+    // A v_i
+    float av[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            float s = A.a[r][0] * V.a[0][i];
+            s = fmaf(A.a[r][1], V.a[1][i], s);
+            s = fmaf(A.a[r][2], V.a[2][i], s);
+            av[i][r] = s;
+        }
+    float u1[3], u2[3], u3[3];
+    {
+        const float n2 = fmaf(av[0][0], av[0][0], fmaf(av[0][1], av[0][1], av[0][2] * av[0][2]));
+        const bool ok = n2 > 0.0f;
+        const float inv = ok ? fast_rsq(n2) : 0.0f;
+        u1[0] = ok ? av[0][0] * inv : 1.0f;
+        u1[1] = ok ? av[0][1] * inv : 0.0f;
+        u1[2] = ok ? av[0][2] * inv : 0.0f;
+    }
+    {
+        const float d = fmaf(u1[0], av[1][0], fmaf(u1[1], av[1][1], u1[2] * av[1][2]));
+        float w[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) w[r] = fmaf(-d, u1[r], av[1][r]);
+        const float n2 = fmaf(w[0], w[0], fmaf(w[1], w[1], w[2] * w[2]));
+        if (n2 > 0.0f) {
+            const float inv = fast_rsq(n2);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) u2[r] = w[r] * inv;
+        } else {  // rank <= 1: any unit vector orthogonal to u1
+            const bool ex = fabsf(u1[0]) < 0.9f;
+            const float e[3] = {ex ? 1.0f : 0.0f, ex ? 0.0f : 1.0f, 0.0f};
+            const float de = fmaf(u1[0], e[0], u1[1] * e[1]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) w[r] = fmaf(-de, u1[r], e[r]);
+            const float inv = fast_rsq(fmaf(w[0], w[0], fmaf(w[1], w[1], w[2] * w[2])));
+#pragma unroll
+            for (int r = 0; r < 3; ++r) u2[r] = w[r] * inv;
+        }
+    }
+    u3[0] = fmaf(u1[1], u2[2], -u1[2] * u2[1]);
+    u3[1] = fmaf(u1[2], u2[0], -u1[0] * u2[2]);
+    u3[2] = fmaf(u1[0], u2[1], -u1[1] * u2[0]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        U.a[r][0] = u1[r];
+        U.a[r][1] = u2[r];
+        U.a[r][2] = u3[r];
+    }
+    sig[0] = fmaf(u1[0], av[0][0], fmaf(u1[1], av[0][1], u1[2] * av[0][2]));
+    sig[1] = fmaf(u2[0], av[1][0], fmaf(u2[1], av[1][1], u2[2] * av[1][2]));
+    sig[2] = fmaf(u3[0], av[2][0], fmaf(u3[1], av[2][1], u3[2] * av[2][2]));
+}
+
+// The SVD of the step: nclr_svd's operation order when EXACT (bitwise the oracle's FP32
+// arithmetic), else the fast 3x3 path (2x2 always takes nclr_svd, a single Jacobi pair).
+template <bool EXACT, int D>
+NCLR_DI void step_svd(const M<D>& A, M<D>& U, float (&sig)[D], M<D>& V) {
+    if constexpr (D == 3 && !EXACT) fast_svd3(A, U, sig, V);
+    else nclr_svd<D>(A, U, sig, V);
+}
+
 // Rotation factor of nclr_polar (src/nclr_math.h:76-98).
 NCLR_DI M<2> polar_rotation(const M<2>& A) {
     const float x = A.a[0][0] + A.a[1][1];
@@ -257,11 +405,21 @@ NCLR_DI M<2> polar_rotation(const M<2>& A) {
     return R;
 }
 
+template <bool EXACT = true>
 NCLR_DI M<3> polar_rotation(const M<3>& A) {
     M<3> U, V;
     float sig[3];
-    nclr_svd<3>(A, U, sig, V);
+    step_svd<EXACT, 3>(A, U, sig, V);
     return matmul_bt<3>(U, V);
+}
+
+template <bool EXACT>
+NCLR_DI M<2> polar_rotation_mode(const M<2>& A) {
+    return polar_rotation(A);
+}
+template <bool EXACT>
+NCLR_DI M<3> polar_rotation_mode(const M<3>& A) {
+    return polar_rotation<EXACT>(A);
 }
 
 // Quadratic B-spline weights (src/nclr.h:124-127).

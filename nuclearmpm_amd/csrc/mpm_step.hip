@@ -1,169 +1,60 @@
 // HIP kernels of the MLS-MPM step for gfx950 (MI355X).
 //
 // Reference step: MPMSimulation<dim>::advance() = p2g(); grid_op(); g2p(); (src/nclr.h:80-84).
+// One launch sequence advances every scene of a batch (a single simulation is a batch of one).
 //
 // Device data layout
-//   particles   structure-of-arrays, field f of slot i at st[f*stride + i] (Layout<D>), in two
-//               ping-pong buffers; pid[i] is the original particle index of slot i.
+//   particles   structure-of-arrays, field f of slot i at st[f*stride + i] (Layout<D>), two
+//               ping-pong buffers; pid[i] is the original particle index of slot i, scenes own
+//               consecutive original indices.
 //   tiles       particles are keyed by the tile of their stencil base cell (S^D cells, S = 16 in
-//               2D, 8 in 3D) and counting-sorted every step (mpm_sort.hip), giving perm:
-//               sorted position -> slot, and [tile_start, tile_end) per tile.
-//   slots       per particle tile, its P2G region of (S+2)^D nodes as float4 {mom, m}.
-//   gv          the updated grid, node tiles of S^D nodes, float4 {v, m}; stamps[node tile] ==
-//               current stamp marks the tiles this step touched.
+//               2D, 8 in 3D), tiles numbered scene after scene, and counting-sorted every step
+//               (mpm_sort.hip): perm maps sorted position -> slot, [tile_start, tile_end) per tile.
+//   slots       per P2G work item (a chunk of one tile), its region of (S+2)^D nodes as float4
+//               {momentum, mass}.
+//   gv          the updated grid in node tiles of S^D nodes, float4 {v, m}; stamps[node tile] ==
+//               *stamp marks the node tiles the current step touched.
 //
 // Kernels per step
-//   keys        tile key per particle (out-of-grid particles get key ntiles and set the status)
-//   sort        stable radix counting sort of (key, slot), tile bounds, P2G work items
-//   p2g         one workgroup per (tile, chunk of <= CHUNK particles): LDS scatter of the
-//               3^D stencil, region written to the tile's slot (plain stores; atomics when the
-//               tile has several chunks)
-//   grid        one workgroup per node tile: sums the <= 2^D slots covering each node and
-//               applies grid_op (src/nclr.h:263-310)
-//   g2p         per sorted position j: gathers particle perm[j], writes it to slot j of the
-//               other buffer, so the state stays physically sorted by tile
+//   keys   tile key per particle; bumps the device step stamp; clears the tile bounds
+//   sort   stable radix counting sort of (key, slot); tile bounds; P2G work items
+//   p2g    one workgroup per work item: fused affine matrix per particle, quantised 3^D
+//          stencil contributions summed exactly with 64-bit LDS integer atomics, region -> slot
+//   grid   one workgroup per node tile: sums the slots covering each node, grid_op
+//          (src/nclr.h:263-310)
+//   g2p    per sorted position j: particle perm[j] -> slot j of the other buffer (the state stays
+//          physically sorted by tile)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
-#include "mpm_internal.hpp"
-#include "mpm_math.hpp"
+#include "mpm_device.hpp"
 
 namespace nclr_dev {
-
-template <int D>
-struct Layout {
-    static constexpr int X = 0;
-    static constexpr int V = D;
-    static constexpr int F = 2 * D;
-    static constexpr int C = 2 * D + D * D;
-    static constexpr int JP = 2 * D + 2 * D * D;
-    static constexpr int MASS = JP + 1;
-    static constexpr int VOL = JP + 2;
-    static constexpr int NF = JP + 3;
-};
-
-// base = trunc(x*inv_dx - 0.5), fx = x*inv_dx - base (src/nclr.h:115,121), with the stencil
-// nodes base..base+2 required to lie in [0, res] on every axis.
-template <int D>
-NCLR_DI bool stencil(const float (&x)[D], const StepParams& p, int (&base)[D], float (&fx)[D], float (&w)[D][3]) {
-    bool ok = true;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        const float s = x[a] * p.inv_dx;
-        const float b = s - 0.5f;
-        ok = ok && (b > -1.0f) && (b < float(p.res - 1));
-        base[a] = ok ? int(b) : 0;
-        fx[a] = s - float(base[a]);
-        bspline_weights(fx[a], w[a]);
-    }
-    return ok;
-}
-
-NCLR_DI float ldf(const float* __restrict__ st, int64_t stride, int f, int64_t i) { return st[int64_t(f) * stride + i]; }
-
-template <int MAT>
-NCLR_DI float hardening(float Jp) {
-    if constexpr (MAT == MPM_SNOW) {
-        return float(exp(10.0 * (1.0 - double(Jp))));
-    } else if constexpr (MAT == MPM_JELLY) {
-        return 0.3f;
-    } else {
-        return 1.0f;
-    }
-}
-
-// first_piola_kirchoff_stress (src/nclr.h:313-337): -(dt V0)(4 inv_dx^2 PF) + m C with
-// PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry.
-template <int D, int MAT>
-NCLR_DI M<D> fused_affine(const M<D>& F, const M<D>& C, float Jp, float mass, float vol, const StepParams& p) {
-    const float e = hardening<MAT>(Jp);
-    const float mu = p.mu0 * e;
-    const float lambda = p.lambda0 * e;
-    const float J = det(F);
-    const M<D> R = polar_rotation(F);
-    const float Dinv = 4.0f * p.inv_dx * p.inv_dx;
-    const float two_mu = 2.0f * mu;
-    M<D> FmR;
-#pragma unroll
-    for (int a = 0; a < D; ++a)
-#pragma unroll
-        for (int b = 0; b < D; ++b) FmR.a[a][b] = two_mu * (F.a[a][b] - R.a[a][b]);
-    const M<D> PF = matmul_bt<D>(FmR, F);
-    const float lam_term = lambda * (J - 1.0f) * J;
-    const float neg_dtv = -(p.dt * vol);
-    M<D> A;
-#pragma unroll
-    for (int a = 0; a < D; ++a)
-#pragma unroll
-        for (int b = 0; b < D; ++b) A.a[a][b] = neg_dtv * (Dinv * (PF.a[a][b] + lam_term)) + mass * C.a[a][b];
-    return A;
-}
-
-template <int D>
-NCLR_DI void offsets(int o, int (&off)[D]) {
-    if constexpr (D == 2) {
-        off[0] = o / 3;
-        off[1] = o % 3;
-    } else {
-        off[0] = o / 9;
-        off[1] = (o / 3) % 3;
-        off[2] = o % 3;
-    }
-}
-
-// Index of node n in the tile-blocked updated grid.
-template <int D>
-NCLR_DI int64_t gv_index(const int (&n)[D], int ntn) {
-    using TL = Tile<D>;
-    int64_t t = 0;
-    int local = 0;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        t = t * ntn + (n[a] >> TL::LOG);
-        local = local * TL::S + (n[a] & (TL::S - 1));
-    }
-    return t * TL::NODES + local;
-}
-
-template <int D>
-NCLR_DI uint32_t tile_key(const int (&base)[D], int nt) {
-    uint32_t t = 0;
-#pragma unroll
-    for (int a = 0; a < D; ++a) t = t * uint32_t(nt) + uint32_t(base[a] >> Tile<D>::LOG);
-    return t;
-}
-
-template <int D>
-NCLR_DI void tile_coords(uint32_t t, int n, int (&c)[D]) {
-#pragma unroll
-    for (int a = D - 1; a >= 0; --a) {
-        c[a] = int(t % uint32_t(n));
-        t /= uint32_t(n);
-    }
-}
 
 // ---------------------------------------------------------------------------------------------
 // keys + tile bounds + work items
 // ---------------------------------------------------------------------------------------------
-// Tile key of every particle; also clears the tile bounds that k_tile_bounds fills after the sort.
 template <int D>
-__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ st, StepParams p, uint32_t* __restrict__ keys,
-                                              uint32_t* __restrict__ status, uint32_t* __restrict__ tile_start,
-                                              uint32_t* __restrict__ tile_end) {
+__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ st, const uint32_t* __restrict__ pid,
+                                              StepParams p, uint32_t* __restrict__ stamp,
+                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ status,
+                                              uint32_t* __restrict__ tile_start, uint32_t* __restrict__ tile_end) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) *stamp = *stamp + 1u;
     if (i <= int64_t(p.ntiles)) {
         tile_start[i] = 0;
         tile_end[i] = 0;
     }
     if (i >= p.n) return;
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_particle(p, pid[i])];
     float x[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) x[a] = ldf(st, p.stride, Layout<D>::X + a, i);
     int base[D];
     float fx[D], w[D][3];
-    if (stencil<D>(x, p, base, fx, w)) {
-        keys[i] = tile_key<D>(base, p.nt);
+    if (stencil<D>(x, sp, base, fx, w)) {
+        keys[i] = sp.tile_base + tile_key<D>(base, sp.nt);
     } else {
         keys[i] = p.ntiles;
         atomicOr(status, kStatusOutOfRange);
@@ -179,11 +70,10 @@ __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict_
     if (j == n - 1 || keys[j + 1] != k) end[k] = uint32_t(j + 1);
 }
 
-// P2G work items: ceil(count / CHUNK) per tile, exclusive scan into item_offset[0..ntiles]
+// P2G work items: ceil(count / chunk) per tile, exclusive scan into item_offset[0..ntiles]
 // (item_offset[ntiles] = total).  One workgroup of 1024 threads, coalesced rounds of 4096 tiles.
-template <int D>
 __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                                                uint32_t ntiles, uint32_t* __restrict__ item_offset) {
+                                                uint32_t ntiles, uint32_t chunk, uint32_t* __restrict__ item_offset) {
     __shared__ uint32_t part[1024];
     const int t = threadIdx.x;
     uint32_t carry = 0;
@@ -193,7 +83,7 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ sta
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t tile = b + uint32_t(t) * 4 + k;
-            v[k] = tile < ntiles ? (end[tile] - start[tile] + Tile<D>::CHUNK - 1) / Tile<D>::CHUNK : 0u;
+            v[k] = tile < ntiles ? (end[tile] - start[tile] + chunk - 1) / chunk : 0u;
             s += v[k];
         }
         part[t] = s;
@@ -217,9 +107,9 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ sta
 }
 
 // ---------------------------------------------------------------------------------------------
-// P2G (src/nclr.h:104-165): LDS tile scatter, region -> slot
+// P2G (src/nclr.h:104-165)
 // ---------------------------------------------------------------------------------------------
-template <int D, int MAT>
+template <int D, int MAT, int PPT, bool EXACT>
 __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, const uint32_t* __restrict__ perm,
                                                   const uint32_t* __restrict__ tile_start,
                                                   const uint32_t* __restrict__ tile_end,
@@ -228,7 +118,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
     using TL = Tile<D>;
     using L = Layout<D>;
     constexpr int REG = TL::REGION;
-    constexpr int PPT = TL::CHUNK / 256;  // particles per thread
+    constexpr int CHUNK = 256 * PPT;
     // LDS accumulators: exact fixed-point sums (64-bit integer atomics are an order of magnitude
     // faster than float atomics in LDS on gfx950, and make the sum independent of lane order)
     __shared__ unsigned long long acc[4 * REG];
@@ -237,26 +127,19 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
     __shared__ uint32_t s_tile, s_chunk;
     const uint32_t item = blockIdx.x;
     if (item >= item_offset[p.ntiles]) return;
-    if (threadIdx.x == 0) {
-        uint32_t lo = 0, hi = p.ntiles;  // item_offset[lo] <= item < item_offset[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (item_offset[mid] <= item) lo = mid;
-            else hi = mid;
-        }
-        s_tile = lo;
-        s_chunk = item - item_offset[lo];
-    }
+    if (threadIdx.x == 0) item_tile(item_offset, p.ntiles, item, s_tile, s_chunk);
     for (int r = threadIdx.x; r < 4 * REG; r += blockDim.x) acc[r] = 0ull;
     for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_tile(p, tile)];
+    const int material = MAT == kMixed ? sp.material : MAT;
     int torg[D];
-    tile_coords<D>(tile, p.nt, torg);
+    tile_coords<D>(tile - sp.tile_base, sp.nt, torg);
 #pragma unroll
     for (int a = 0; a < D; ++a) torg[a] *= TL::S;
-    const uint32_t j0 = tile_start[tile] + s_chunk * TL::CHUNK;
-    const uint32_t j1 = min(j0 + uint32_t(TL::CHUNK), tile_end[tile]);
+    const uint32_t j0 = tile_start[tile] + s_chunk * CHUNK;
+    const uint32_t j1 = min(j0 + uint32_t(CHUNK), tile_end[tile]);
 
     // pass 1: per particle the fused affine matrix, kept in registers, and a bound of the sum of
     // |contributions| per channel: sum_nodes |w| (|m v_c| + 2.5 dx sum_k |A_ck|)
@@ -292,9 +175,9 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
         const float vol = ldf(st, p.stride, L::VOL, i);
         int base[D];
         float fx[D], w[D][3];
-        if (!stencil<D>(x, p, base, fx, w)) continue;  // keyed into this tile, so never taken
+        if (!stencil<D>(x, sp, base, fx, w)) continue;  // keyed into this tile, so never taken
         valid[k] = true;
-        As[k] = fused_affine<D, MAT>(F, C, Jp, mass, vol, p);
+        As[k] = fused_affine<D, EXACT>(material, F, C, Jp, mass, vol, sp);
         ms[k] = mass;
         float wsum = 1.0f;
 #pragma unroll
@@ -309,7 +192,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
             float s = 0.0f;
 #pragma unroll
             for (int b = 0; b < D; ++b) s += fabsf(As[k].a[c][b]);
-            const float bc = wsum * (fabsf(mvs[k][c]) + 2.5f * p.dx * s);
+            const float bc = wsum * (fabsf(mvs[k][c]) + 2.5f * sp.dx * s);
             if (bc < 3.0e38f) bound[c] += bc;  // non-finite contributions take the flag path
         }
         const float bm = wsum * fabsf(mass);
@@ -331,11 +214,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const float total = (s_bound[0][c] + s_bound[1][c]) + (s_bound[2][c] + s_bound[3][c]);
-        int ex = 0;
-        (void)frexpf(total > 0.0f && total < 3.0e38f ? total : 1.0f, &ex);  // total < 2^ex
-        const int e = min(127, 61 - ex);                                      // in [-68, 127]
-        scale[c] = ldexpf(1.0f, e);
-        inv_scale[c] = ldexpf(1.0f, -e);
+        fixed_point_scale(total, scale[c], inv_scale[c]);
     }
 
     // pass 2: quantised contributions -> 64-bit integer LDS atomics
@@ -355,7 +234,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
             int ro = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                dpos[a] = (float(off[a]) - fxs[k][a]) * p.dx;
+                dpos[a] = (float(off[a]) - fxs[k][a]) * sp.dx;
                 ro = ro * TL::R + off[a];
             }
             float weight = w[0][off[0]];
@@ -414,8 +293,9 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
     __shared__ int s_count[NC];  // number of work items of candidate tile c
     __shared__ uint32_t s_active;
     const uint32_t nt_idx = blockIdx.x;
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_node_tile(p, nt_idx)];
     int nto[D];
-    tile_coords<D>(nt_idx, p.ntn, nto);
+    tile_coords<D>(nt_idx - sp.node_tile_base, sp.ntn, nto);
     bool occupied = false;
     if (threadIdx.x == 0) s_active = 0;
     if (threadIdx.x < NC) {
@@ -424,11 +304,12 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             const int ta = nto[a] - ((threadIdx.x >> a) & 1);
-            valid = valid && ta >= 0 && ta < p.nt;
-            t = t * p.nt + ta;
+            valid = valid && ta >= 0 && ta < sp.nt;
+            t = t * sp.nt + ta;
         }
-        const uint32_t first = valid ? item_offset[t] : 0u;
-        const uint32_t count = valid ? item_offset[t + 1] - first : 0u;
+        const uint32_t gt = sp.tile_base + uint32_t(t);
+        const uint32_t first = valid ? item_offset[gt] : 0u;
+        const uint32_t count = valid ? item_offset[gt + 1] - first : 0u;
         occupied = count > 0;
         s_first[threadIdx.x] = occupied ? int(first) : -1;
         s_count[threadIdx.x] = int(count);
@@ -437,7 +318,8 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
         if (threadIdx.x == 0) active[nt_idx] = 0;
         return;
     }
-    if (threadIdx.x == 0) stamps[nt_idx] = p.stamp;
+    const uint32_t stamp = *p.stamp;
+    if (threadIdx.x == 0) stamps[nt_idx] = stamp;
     int first[NC], count[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -485,16 +367,16 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
         if (m > 0.0f) {
 #pragma unroll
             for (int a = 0; a < D; ++a) vel[a] = vel[a] / m;
-            vel[1] = vel[1] + p.dt * p.gravity;
+            vel[1] = vel[1] + sp.dt * sp.gravity;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                const float lo = -p.allowed, hi = p.allowed;
+                const float lo = -sp.allowed, hi = sp.allowed;
                 vel[a] = (vel[a] < lo) ? lo : ((hi < vel[a]) ? hi : vel[a]);
             }
         }
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            if ((n[a] < 3 && vel[a] < 0.0f) || (n[a] >= (p.res + 1) - 3 && vel[a] > 0.0f)) {
+            if ((n[a] < 3 && vel[a] < 0.0f) || (n[a] >= (sp.res + 1) - 3 && vel[a] > 0.0f)) {
 #pragma unroll
                 for (int b = 0; b < D; ++b) vel[b] = 0.0f;
                 m = 0.0f;
@@ -508,7 +390,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
     if (threadIdx.x == 0) active[nt_idx] = s_active;
 }
 
-// Sum of n counters into *out (one workgroup).
+// Sum of v[first .. first+n) into *out (one workgroup).
 __global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t* __restrict__ v, int64_t n,
                                                   unsigned long long* __restrict__ out) {
     __shared__ unsigned long long part[1024];
@@ -526,7 +408,7 @@ __global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t* __restrict__ v
 // ---------------------------------------------------------------------------------------------
 // G2P (src/nclr.h:167-261), reading slot perm[j] and writing slot j of the other buffer
 // ---------------------------------------------------------------------------------------------
-template <int D, int MAT>
+template <int D, int MAT, bool EXACT>
 __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st, float* __restrict__ out,
                                                     const uint32_t* __restrict__ pid_in,
                                                     uint32_t* __restrict__ pid_out, const uint32_t* __restrict__ perm,
@@ -536,7 +418,10 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
     if (j >= p.n) return;
     const int64_t i = perm[j];
     const int64_t s = p.stride;
-    pid_out[j] = pid_in[i];
+    const uint32_t id = pid_in[i];
+    pid_out[j] = id;
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_particle(p, id)];
+    const int material = MAT == kMixed ? sp.material : MAT;
     float x[D];
     M<D> F;
 #pragma unroll
@@ -550,7 +435,7 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
     out[int64_t(L::VOL) * s + j] = ldf(st, s, L::VOL, i);
     int base[D];
     float fx[D], w[D][3];
-    if (!stencil<D>(x, p, base, fx, w)) {
+    if (!stencil<D>(x, sp, base, fx, w)) {
         // outside the grid (already reported by the key kernel): carried through unchanged
 #pragma unroll
         for (int f = 0; f < L::JP + 1; ++f) out[int64_t(f) * s + j] = ldf(st, s, f, i);
@@ -564,7 +449,7 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
 #pragma unroll
         for (int b = 0; b < D; ++b) C.a[a][b] = 0.0f;
     }
-    const float four_inv_dx = 4.0f * p.inv_dx;
+    const float four_inv_dx = 4.0f * sp.inv_dx;
     constexpr int K = D == 2 ? 9 : 27;
 #pragma unroll
     for (int o = 0; o < K; ++o) {
@@ -577,7 +462,7 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
             dpos[a] = float(off[a]) - fx[a];
             nidx[a] = base[a] + off[a];
         }
-        const float4 g = gv[gv_index<D>(nidx, p.ntn)];
+        const float4 g = gv[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
         const float gvel[3] = {g.x, g.y, g.z};
         float weight = w[0][off[0]];
 #pragma unroll
@@ -592,23 +477,23 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
         }
     }
 #pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = x[a] + p.dt * v[a];
+    for (int a = 0; a < D; ++a) x[a] = x[a] + sp.dt * v[a];
     M<D> Mx;
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
         for (int b = 0; b < D; ++b) {
-            const float id = (a == b && (a < 2 || p.identity_mode == 1)) ? 1.0f : 0.0f;
-            Mx.a[a][b] = id + p.dt * C.a[a][b];
+            const float id1 = (a == b && (a < 2 || sp.identity_mode == 1)) ? 1.0f : 0.0f;
+            Mx.a[a][b] = id1 + sp.dt * C.a[a][b];
         }
     M<D> Fh = matmul<D>(Mx, F);
-    if constexpr (MAT == MPM_JELLY) {
+    if (material == MPM_JELLY) {
         F = Fh;
     } else {
         M<D> U, V;
         float sig[D];
-        nclr_svd<D>(Fh, U, sig, V);
-        if constexpr (MAT == MPM_SNOW) {
+        step_svd<EXACT, D>(Fh, U, sig, V);
+        if (material == MPM_SNOW) {
             const float lo = float(1.0 - 2.5e-2), hi = float(1.0 + 4.5e-3);
 #pragma unroll
             for (int d = 0; d < D; ++d) sig[d] = (sig[d] < lo) ? lo : ((hi < sig[d]) ? hi : sig[d]);
@@ -629,7 +514,8 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
 #pragma unroll
             for (int a = 0; a < D; ++a)
 #pragma unroll
-                for (int b = 0; b < D; ++b) F.a[a][b] = (a == b && (a < 2 || p.identity_mode == 1)) ? 1.0f : 0.0f;
+                for (int b = 0; b < D; ++b)
+                    F.a[a][b] = (a == b && (a < 2 || sp.identity_mode == 1)) ? 1.0f : 0.0f;
             F.a[0][0] = float(J);
         }
     }
@@ -648,14 +534,15 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
         }
 }
 
-// Dense export of the updated grid in the reference's flat x-major order (src/nclr.h:87); nodes of
-// node tiles the last step did not touch are empty cells (zero).
+// Dense export of one scene's updated grid in the reference's flat x-major order
+// (src/nclr.h:87); nodes of node tiles the last step did not touch are empty cells (zero).
 template <int D>
 __global__ __launch_bounds__(256) void k_export_grid(const float4* __restrict__ gv, const uint32_t* __restrict__ stamps,
-                                                     StepParams p, float* __restrict__ vel,
+                                                     const uint32_t* __restrict__ stamp, int res,
+                                                     uint32_t node_tile_base, int ntn, float* __restrict__ vel,
                                                      float* __restrict__ mass) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t side = p.res + 1;
+    const int64_t side = res + 1;
     int64_t total = side * side;
     if constexpr (D == 3) total *= side;
     if (idx >= total) return;
@@ -666,13 +553,38 @@ __global__ __launch_bounds__(256) void k_export_grid(const float4* __restrict__ 
         n[a] = int(r % side);
         r /= side;
     }
-    const int64_t node = gv_index<D>(n, p.ntn);
-    const bool live = p.stamp != 0 && stamps[node / Tile<D>::NODES] == p.stamp;
+    const int64_t node = gv_index<D>(n, ntn, node_tile_base);
+    const uint32_t now = *stamp;
+    const bool live = now != 0 && stamps[node / Tile<D>::NODES] == now;
     const float4 g = live ? gv[node] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     vel[idx * D + 0] = g.x;
     vel[idx * D + 1] = g.y;
     if constexpr (D == 3) vel[idx * D + 2] = g.z;
     mass[idx] = g.w;
+}
+
+// The step's SVD on a batch of row-major matrices.
+template <int D, bool EXACT>
+__global__ __launch_bounds__(256) void k_svd(const float* __restrict__ A, float* __restrict__ U,
+                                             float* __restrict__ S, float* __restrict__ V, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    M<D> a, u, v;
+    float s[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.a[r][c] = A[(i * D + r) * D + c];
+    step_svd<EXACT, D>(a, u, s, v);
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        S[i * D + r] = s[r];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            U[(i * D + r) * D + c] = u.a[r][c];
+            V[(i * D + r) * D + c] = v.a[r][c];
+        }
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t value) {
@@ -712,40 +624,61 @@ static hipError_t sort_dispatch(const LaunchArgs& a) {
     const auto& p = a.p;
     const int64_t nb = int64_t(p.ntiles) + 1;
     hipStream_t s = a.stream;
-    k_keys<D><<<blocks_for(std::max<int64_t>(p.n, nb), 256), 256, 0, s>>>(a.state_in, p, a.keys, a.status,
-                                                                           a.tile_start, a.tile_end);
+    k_keys<D><<<blocks_for(std::max<int64_t>(std::max<int64_t>(p.n, nb), 1), 256), 256, 0, s>>>(
+        a.state_in, a.pid_in, p, a.stamp, a.keys, a.status, a.tile_start, a.tile_end);
     if (p.n > 0) {
         hipError_t e = radix_sort_pairs(a.keys, a.perm, a.keys_tmp, a.perm_tmp, a.hist, p.n, key_bits(p.ntiles),
                                         true, s);
         if (e != hipSuccess) return e;
         k_tile_bounds<<<blocks_for(p.n, 256), 256, 0, s>>>(a.keys, p.n, a.tile_start, a.tile_end);
     }
-    k_items<D><<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, a.item_offset);
+    k_items<<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset);
+    return hipGetLastError();
+}
+
+template <int D, int MAT, int PPT>
+static hipError_t p2g_one(const LaunchArgs& a) {
+    const int64_t items = (a.p.n + 256 * PPT - 1) / (256 * PPT) + a.p.ntiles;
+    if (D == 3 && !a.exact_svd) {
+        k_p2g_tile<D, MAT, PPT, false><<<unsigned(items), 256, 0, a.stream>>>(
+            a.state_in, a.perm, a.tile_start, a.tile_end, a.item_offset, a.p, a.slots);
+        return hipGetLastError();
+    }
+    k_p2g_tile<D, MAT, PPT, true><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.tile_start, a.tile_end,
+                                                                   a.item_offset, a.p, a.slots);
     return hipGetLastError();
 }
 
 template <int D, int MAT>
-static hipError_t p2g_one(const LaunchArgs& a) {
-    const int64_t items = (a.p.n + Tile<D>::CHUNK - 1) / Tile<D>::CHUNK + a.p.ntiles;
-    k_p2g_tile<D, MAT><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.tile_start, a.tile_end,
-                                                              a.item_offset, a.p, a.slots);
-    return hipGetLastError();
+static hipError_t p2g_chunk(const LaunchArgs& a) {
+    switch (a.p.chunk) {
+        case 256: return p2g_one<D, MAT, 1>(a);
+        case 1024: return p2g_one<D, MAT, 4>(a);
+        default: return p2g_one<D, MAT, 2>(a);
+    }
 }
 
 template <int D>
 static hipError_t p2g_dispatch(const LaunchArgs& a) {
     if (a.p.n == 0) return hipSuccess;
     switch (a.material) {
-        case MPM_SNOW: return p2g_one<D, MPM_SNOW>(a);
-        case MPM_JELLY: return p2g_one<D, MPM_JELLY>(a);
-        default: return p2g_one<D, MPM_LIQUID>(a);
+        case MPM_SNOW: return p2g_chunk<D, MPM_SNOW>(a);
+        case MPM_JELLY: return p2g_chunk<D, MPM_JELLY>(a);
+        case MPM_LIQUID: return p2g_chunk<D, MPM_LIQUID>(a);
+        default: return p2g_chunk<D, kMixed>(a);
     }
 }
 
 template <int D, int MAT>
 static hipError_t g2p_one(const LaunchArgs& a) {
-    k_g2p_sorted<D, MAT><<<blocks_for(a.p.n, 256), 256, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in, a.pid_out,
-                                                                       a.perm, a.p, a.gv);
+    const int threads = a.g2p_threads > 0 ? a.g2p_threads : 256;
+    if (D == 3 && !a.exact_svd) {
+        k_g2p_sorted<D, MAT, false><<<blocks_for(a.p.n, threads), threads, 0, a.stream>>>(
+            a.state_in, a.state_out, a.pid_in, a.pid_out, a.perm, a.p, a.gv);
+        return hipGetLastError();
+    }
+    k_g2p_sorted<D, MAT, true><<<blocks_for(a.p.n, threads), threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in,
+                                                                              a.pid_out, a.perm, a.p, a.gv);
     return hipGetLastError();
 }
 
@@ -755,7 +688,8 @@ static hipError_t g2p_dispatch(const LaunchArgs& a) {
     switch (a.material) {
         case MPM_SNOW: return g2p_one<D, MPM_SNOW>(a);
         case MPM_JELLY: return g2p_one<D, MPM_JELLY>(a);
-        default: return g2p_one<D, MPM_LIQUID>(a);
+        case MPM_LIQUID: return g2p_one<D, MPM_LIQUID>(a);
+        default: return g2p_one<D, kMixed>(a);
     }
 }
 
@@ -764,32 +698,38 @@ hipError_t launch_p2g(const LaunchArgs& a) { return a.dim == 2 ? p2g_dispatch<2>
 hipError_t launch_g2p(const LaunchArgs& a) { return a.dim == 2 ? g2p_dispatch<2>(a) : g2p_dispatch<3>(a); }
 
 hipError_t launch_grid(const LaunchArgs& a) {
-    int64_t tiles = 1;
-    for (int d = 0; d < a.dim; ++d) tiles *= a.p.ntn;
+    const unsigned tiles = a.p.node_tiles;
+    if (tiles == 0) return hipSuccess;
     if (a.dim == 2)
-        k_grid_tiles<2><<<unsigned(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+        k_grid_tiles<2><<<tiles, 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
     else
-        k_grid_tiles<3><<<unsigned(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+        k_grid_tiles<3><<<tiles, 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
     return hipGetLastError();
 }
 
-hipError_t launch_active_sum(const LaunchArgs& a, unsigned long long* out) {
-    int64_t tiles = 1;
-    for (int d = 0; d < a.dim; ++d) tiles *= a.p.ntn;
-    k_sum_u32<<<1, 1024, 0, a.stream>>>(a.active, tiles, out);
+hipError_t launch_active_sum(const LaunchArgs& a, uint32_t first_tile, uint32_t tiles, unsigned long long* out) {
+    k_sum_u32<<<1, 1024, 0, a.stream>>>(a.active + first_tile, int64_t(tiles), out);
     return hipGetLastError();
 }
 
-int64_t max_items(int dim, int64_t n, uint32_t ntiles) {
-    const int64_t chunk = dim == 2 ? Tile<2>::CHUNK : Tile<3>::CHUNK;
-    return (n + chunk - 1) / chunk + int64_t(ntiles);
+hipError_t launch_export_grid(const LaunchArgs& a, int scene, int res, uint32_t node_tile_base, int ntn, float* vel,
+                              float* mass, int64_t total) {
+    (void)scene;
+    if (a.dim == 2)
+        k_export_grid<2><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.stamp, res, node_tile_base,
+                                                                       ntn, vel, mass);
+    else
+        k_export_grid<3><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.stamp, res, node_tile_base,
+                                                                       ntn, vel, mass);
+    return hipGetLastError();
 }
 
-hipError_t launch_export_grid(const LaunchArgs& a, float* vel, float* mass, int64_t total) {
-    if (a.dim == 2)
-        k_export_grid<2><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.p, vel, mass);
-    else
-        k_export_grid<3><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.p, vel, mass);
+hipError_t launch_svd(int dim, bool exact, const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned b = blocks_for(n, 256);
+    if (dim == 2) k_svd<2, true><<<b, 256, 0, s>>>(A, U, S, V, n);
+    else if (exact) k_svd<3, true><<<b, 256, 0, s>>>(A, U, S, V, n);
+    else k_svd<3, false><<<b, 256, 0, s>>>(A, U, S, V, n);
     return hipGetLastError();
 }
 
@@ -811,6 +751,7 @@ hipError_t launch_zero_f4(float4* p, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+int64_t max_items(int64_t n, uint32_t ntiles) { return (n + kMinChunk - 1) / kMinChunk + int64_t(ntiles); }
 int num_fields(int dim) { return dim == 2 ? Layout<2>::NF : Layout<3>::NF; }
 int region_nodes(int dim) { return dim == 2 ? Tile<2>::REGION : Tile<3>::REGION; }
 int tile_nodes(int dim) { return dim == 2 ? Tile<2>::NODES : Tile<3>::NODES; }

@@ -6,6 +6,10 @@
     sim.advance()
     state = sim.particles()          # numpy arrays, original particle order
 
+MPMBatch advances many independent scenes (uneven particle counts, any materials, per-scene
+res/dt/E/nu/gravity) with one launch sequence per step; scene s behaves exactly like an
+MPMSimulation built from the same arguments.
+
 Differences from the reference, all deliberate:
   * particles() returns a Particles record of numpy arrays (x (n,d), v, F (n,d,d), C, Jp,
     mass, volume, c) instead of a vector of structs;
@@ -19,12 +23,11 @@ from __future__ import annotations
 import ctypes
 from dataclasses import dataclass, field
 from enum import IntEnum
-from typing import Iterable, Optional, Sequence, Union
+from typing import List, Optional, Sequence, Union
 
 import numpy as np
 
-from . import _lib
-from ._lib import check, lib, mpm_options, ptr
+from ._lib import check, lib, mpm_options, mpm_scene, ptr
 
 
 class MaterialModel(IntEnum):
@@ -111,6 +114,14 @@ class Particles:
                                         volume=np.array([p.volume for p in parts], np.float32),
                                         c=np.array([p.c for p in parts], np.int32), identity_mode=identity_mode)
 
+    @staticmethod
+    def concatenate(parts: Sequence["Particles"]) -> "Particles":
+        return Particles(*(np.concatenate([getattr(p, k) for p in parts]) for k in
+                           ("x", "v", "F", "C", "Jp", "mass", "volume", "c")))
+
+    def slice(self, a: int, b: int) -> "Particles":
+        return Particles(*(getattr(self, k)[a:b].copy() for k in ("x", "v", "F", "C", "Jp", "mass", "volume", "c")))
+
     def copy(self) -> "Particles":
         return Particles(*(getattr(self, k).copy() for k in ("x", "v", "F", "C", "Jp", "mass", "volume", "c")))
 
@@ -125,55 +136,27 @@ def default_options(**overrides) -> mpm_options:
     return o
 
 
-class MPMSimulation:
-    """MPMSimulation<dim> (src/nclr.h:63-87) running on the MI355X engine."""
+class _Engine:
+    """Handle management shared by MPMSimulation and MPMBatch."""
 
-    kBoundary = 3
-    kSnowHardening = np.float32(10.0)
-    kJellyHardening = np.float32(0.3)
-    kLiquidHardening = np.float32(1.0)
+    _h = None
 
-    def __init__(self, particles: Union[Particles, Sequence[Particle]], model: MaterialModel, res: int = 64,
-                 dt: float = 1e-4, E: float = 1e4, nu: float = 0.2, gravity: float = -100.0, *,
-                 identity_mode: int = 0, sort_every: int = 0, p2g_variant: int = 0, timing: bool = False,
-                 device: int = 0):
-        if not isinstance(particles, Particles):
-            particles = Particles.from_list(list(particles), identity_mode=identity_mode)
-        self._dim = particles.dim
-        if self._dim not in (2, 3):
-            raise ValueError("dim must be 2 or 3")
-        self._res = int(res)
-        self._model = MaterialModel(model)
-        self._colors = particles.c.copy()
-        self._n = particles.n
-        self.options = default_options(identity_mode=identity_mode, sort_every=sort_every,
-                                       p2g_variant=p2g_variant, timing=int(timing), device=device)
-        h = ctypes.c_void_p()
-        check(lib().mpm_create(self._dim, int(self._model), self._res, dt, E, nu, gravity, self._n,
-                               ptr(particles.x), ptr(particles.v), ptr(particles.F), ptr(particles.C),
-                               ptr(particles.Jp), ptr(particles.mass), ptr(particles.volume),
-                               ctypes.byref(self.options), ctypes.byref(h)))
-        self._h = h
-        mu = ctypes.c_float()
-        lam = ctypes.c_float()
-        dx = ctypes.c_float()
-        inv_dx = ctypes.c_float()
-        check(lib().mpm_get_params(self._h, ctypes.byref(mu), ctypes.byref(lam), ctypes.byref(dx),
-                                   ctypes.byref(inv_dx)))
-        self.mu_0 = np.float32(mu.value)
-        self.lambda_0 = np.float32(lam.value)
-        self.dx = np.float32(dx.value)
-        self.inv_dx = np.float32(inv_dx.value)
+    def _init_common(self, dim: int, n: int, colors, identity_mode, p2g_chunk, g2p_threads, graph, autotune, timing,
+                     device, fast_svd=False):
+        self._dim = dim
+        self._n = n
+        self._colors = colors
+        self.options = default_options(identity_mode=identity_mode, p2g_chunk=p2g_chunk, g2p_threads=g2p_threads,
+                                       graph=int(graph), autotune=int(autotune), timing=int(timing), device=device,
+                                       fast_svd=int(bool(fast_svd)))
         self._cache = None
 
-    # -- reference API ---------------------------------------------------------------------
     def advance(self, steps: int = 1) -> None:
-        """advance() (src/nclr.h:80-84), `steps` times; queued without waiting."""
+        """advance() (src/nclr.h:80-84), `steps` times for every scene; queued without waiting."""
         self._cache = None
         check(lib().mpm_advance(self._handle(), int(steps)))
 
-    def particles(self) -> Particles:
-        """particles() (src/nclr.h:86): the current state, original particle order."""
+    def _all_particles(self) -> Particles:
         if self._cache is None:
             n, d = self._n, self._dim
             out = Particles(x=np.empty((n, d), np.float32), v=np.empty((n, d), np.float32),
@@ -185,18 +168,16 @@ class MPMSimulation:
             self._cache = out
         return self._cache
 
-    def grid(self):
-        """grid() (src/nclr.h:87): (velocity (res+1,)*d + (d,), mass (res+1,)*d) after the last grid update."""
-        nodes = (self._res + 1) ** self._dim
+    def _grid(self, scene: int, res: int):
+        nodes = (res + 1) ** self._dim
         vel = np.empty((nodes, self._dim), np.float32)
         mass = np.empty(nodes, np.float32)
-        check(lib().mpm_get_grid(self._handle(), ptr(vel), ptr(mass)))
-        shape = (self._res + 1,) * self._dim
+        check(lib().mpm_get_grid_scene(self._handle(), int(scene), ptr(vel), ptr(mass)))
+        shape = (res + 1,) * self._dim
         return vel.reshape(shape + (self._dim,)), mass.reshape(shape)
 
-    # -- extras ------------------------------------------------------------------------------
     def set_particles(self, p: Particles) -> None:
-        """Overwrite the particle state (same count and dimension)."""
+        """Overwrite the particle state of all scenes (same count and dimension)."""
         if p.n != self._n or p.dim != self._dim:
             raise ValueError("particle count/dimension mismatch")
         self._cache = None
@@ -207,7 +188,8 @@ class MPMSimulation:
         check(lib().mpm_synchronize(self._handle()))
 
     def kernel_time(self, name: str):
-        """(average ms per launch, launches) of kernel `name` ("sort", "p2g", "grid", "g2p"); needs timing=True."""
+        """(average ms per launch, launches) of kernel group `name` ("sort", "p2g", "grid", "g2p",
+        "step"); needs timing on."""
         avg = ctypes.c_double()
         cnt = ctypes.c_int64()
         check(lib().mpm_kernel_time(self._handle(), name.encode(), ctypes.byref(avg), ctypes.byref(cnt)))
@@ -219,9 +201,34 @@ class MPMSimulation:
     def reset_kernel_times(self) -> None:
         check(lib().mpm_reset_kernel_times(self._handle()))
 
-    def active_nodes(self) -> int:
+    def set_tuning(self, p2g_chunk: int, g2p_threads: int) -> None:
+        check(lib().mpm_set_tuning(self._handle(), int(p2g_chunk), int(g2p_threads)))
+
+    def tuning(self):
+        c = ctypes.c_int()
+        t = ctypes.c_int()
+        check(lib().mpm_get_tuning(self._handle(), ctypes.byref(c), ctypes.byref(t)))
+        return {"p2g_chunk": c.value, "g2p_threads": t.value}
+
+    def autotune(self, steps_per_trial: int = 3):
+        """Time every launch shape on the current state (restored afterwards) and keep the fastest."""
+        check(lib().mpm_autotune(self._handle(), int(steps_per_trial)))
+        return self.tuning()
+
+    def set_elasticity(self, scene: int, E: float, nu: Optional[float] = None) -> None:
+        """Young's modulus (and Poisson's ratio) of one scene; mu_0 and lambda_0 follow in float."""
+        nu = self._nus[scene] if nu is None else float(nu)
+        check(lib().mpm_set_scene_elasticity(self._handle(), int(scene), float(E), nu))
+        self._nus[scene] = nu
+
+    @property
+    def scenes(self) -> int:
+        """Number of scenes the handle advances (1 for a plain simulation)."""
+        return len(self._nus)
+
+    def active_nodes(self, scene: int = -1) -> int:
         c = ctypes.c_int64()
-        check(lib().mpm_active_nodes(self._handle(), ctypes.byref(c)))
+        check(lib().mpm_active_nodes_scene(self._handle(), int(scene), ctypes.byref(c)))
         return c.value
 
     @property
@@ -231,10 +238,6 @@ class MPMSimulation:
     @property
     def dim(self) -> int:
         return self._dim
-
-    @property
-    def res(self) -> int:
-        return self._res
 
     @property
     def n(self) -> int:
@@ -261,3 +264,121 @@ class MPMSimulation:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class MPMSimulation(_Engine):
+    """MPMSimulation<dim> (src/nclr.h:63-87) running on the MI355X engine."""
+
+    kBoundary = 3
+    kSnowHardening = np.float32(10.0)
+    kJellyHardening = np.float32(0.3)
+    kLiquidHardening = np.float32(1.0)
+
+    def __init__(self, particles: Union[Particles, Sequence[Particle]], model: MaterialModel, res: int = 64,
+                 dt: float = 1e-4, E: float = 1e4, nu: float = 0.2, gravity: float = -100.0, *,
+                 identity_mode: int = 0, p2g_chunk: int = 0, g2p_threads: int = 0, graph: bool = False,
+                 autotune: bool = False, timing: bool = False, device: int = 0, fast_svd: bool = False):
+        if not isinstance(particles, Particles):
+            particles = Particles.from_list(list(particles), identity_mode=identity_mode)
+        if particles.dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        self._res = int(res)
+        self._model = MaterialModel(model)
+        self._nus = [float(nu)]
+        self._init_common(particles.dim, particles.n, particles.c.copy(), identity_mode, p2g_chunk, g2p_threads,
+                          graph, autotune, timing, device, fast_svd)
+        h = ctypes.c_void_p()
+        check(lib().mpm_create(self._dim, int(self._model), self._res, dt, E, nu, gravity, self._n,
+                               ptr(particles.x), ptr(particles.v), ptr(particles.F), ptr(particles.C),
+                               ptr(particles.Jp), ptr(particles.mass), ptr(particles.volume),
+                               ctypes.byref(self.options), ctypes.byref(h)))
+        self._h = h
+        mu, lam, dx, inv_dx = (ctypes.c_float() for _ in range(4))
+        check(lib().mpm_get_params(self._h, ctypes.byref(mu), ctypes.byref(lam), ctypes.byref(dx),
+                                   ctypes.byref(inv_dx)))
+        self.mu_0 = np.float32(mu.value)
+        self.lambda_0 = np.float32(lam.value)
+        self.dx = np.float32(dx.value)
+        self.inv_dx = np.float32(inv_dx.value)
+
+    def particles(self) -> Particles:
+        """particles() (src/nclr.h:86): the current state, original particle order."""
+        return self._all_particles()
+
+    def grid(self):
+        """grid() (src/nclr.h:87): (velocity (res+1,)*d + (d,), mass (res+1,)*d) after the last grid update."""
+        return self._grid(0, self._res)
+
+    @property
+    def res(self) -> int:
+        return self._res
+
+
+@dataclass
+class SceneSpec:
+    """One scene of a batch: the reference ctor's arguments."""
+    particles: Particles
+    material: MaterialModel
+    res: int = 64
+    dt: float = 1e-4
+    E: float = 1e4
+    nu: float = 0.2
+    gravity: float = -100.0
+
+
+class MPMBatch(_Engine):
+    """Many independent scenes advanced together: one launch sequence per step for all of them."""
+
+    def __init__(self, scenes: Sequence, *, identity_mode: int = 0, p2g_chunk: int = 0, g2p_threads: int = 0,
+                 graph: bool = False, autotune: bool = False, timing: bool = False, device: int = 0,
+                 fast_svd: bool = False):
+        specs: List[SceneSpec] = []
+        for s in scenes:
+            if isinstance(s, SceneSpec):
+                specs.append(s)
+            else:  # scenes.Scene or anything with the same attributes
+                specs.append(SceneSpec(s.particles, MaterialModel(s.material), s.res, s.dt, s.E, s.nu, s.gravity))
+        if not specs:
+            raise ValueError("a batch needs at least one scene")
+        dims = {sp.particles.dim for sp in specs}
+        if len(dims) != 1:
+            raise ValueError("all scenes of a batch must have the same dimension")
+        self._specs = specs
+        self._nus = [float(sp.nu) for sp in specs]
+        self._offsets = np.cumsum([0] + [sp.particles.n for sp in specs])
+        allp = Particles.concatenate([sp.particles for sp in specs])
+        self._init_common(allp.dim, allp.n, allp.c.copy(), identity_mode, p2g_chunk, g2p_threads, graph, autotune,
+                          timing, device, fast_svd)
+        table = (mpm_scene * len(specs))()
+        for i, sp in enumerate(specs):
+            table[i] = mpm_scene(int(sp.material), int(sp.res), sp.dt, sp.E, sp.nu, sp.gravity, sp.particles.n)
+        h = ctypes.c_void_p()
+        check(lib().mpm_create_batch(self._dim, len(specs), table, ptr(allp.x), ptr(allp.v), ptr(allp.F),
+                                     ptr(allp.C), ptr(allp.Jp), ptr(allp.mass), ptr(allp.volume),
+                                     ctypes.byref(self.options), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def num_scenes(self) -> int:
+        return len(self._specs)
+
+    def scene_range(self, s: int):
+        return int(self._offsets[s]), int(self._offsets[s + 1])
+
+    def particles(self, scene: Optional[int] = None) -> Particles:
+        """All particles (scenes concatenated in order), or those of one scene."""
+        allp = self._all_particles()
+        if scene is None:
+            return allp
+        a, b = self.scene_range(scene)
+        return allp.slice(a, b)
+
+    def grid(self, scene: int):
+        return self._grid(scene, self._specs[scene].res)
+
+    def params(self, scene: int):
+        mu, lam, dx, inv_dx = (ctypes.c_float() for _ in range(4))
+        check(lib().mpm_get_scene_params(self._handle(), int(scene), ctypes.byref(mu), ctypes.byref(lam),
+                                         ctypes.byref(dx), ctypes.byref(inv_dx)))
+        return {"mu_0": np.float32(mu.value), "lambda_0": np.float32(lam.value), "dx": np.float32(dx.value),
+                "inv_dx": np.float32(inv_dx.value)}

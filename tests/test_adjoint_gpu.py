@@ -1,0 +1,255 @@
+"""GPU adjoint of the jelly step (hand-written HIP kernels behind mpm_backward_step /
+mpm_backward_segment and the torch.autograd.Function of nuclearmpm_amd.autodiff).
+
+Pinned against
+  * the FP64 oracle adjoint on the same FP32 inputs (one step, random cotangents),
+  * the FP64 oracle's chained multi-step adjoint, and central finite differences (eps = 1e-3)
+    of the FP64 oracle rollout (an FP32 forward is too noisy for 1e-3 finite differences),
+  * the closed-form gradient of the centre-of-mass loss (momentum-conserving transfers),
+  * checkpoint-interval invariance (k = 1 and k = steps give identical gradients), and
+  * batching (each scene's gradient in a batch equals the scene differentiated alone).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+from nuclearmpm_amd import MaterialModel, MPMBatch, MPMSimulation, Particles, SceneSpec, autodiff, scenes
+from nuclearmpm_amd._lib import check, lib
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "F", "C")
+
+
+def jelly_particles(d, side, seed, lo=0.4, hi=0.6, F_noise=0.05, v_scale=0.5):
+    rng = np.random.default_rng(seed)
+    g = np.linspace(lo, hi, side)
+    x = np.stack(np.meshgrid(*([g] * d), indexing="ij"), -1).reshape(-1, d)
+    x = (x + rng.uniform(-0.01, 0.01, x.shape)).astype(np.float32)
+    n = x.shape[0]
+    p = Particles.from_positions(x)
+    p.v = rng.normal(0, v_scale, (n, d)).astype(np.float32)
+    p.F = (np.eye(d)[None] + rng.normal(0, F_noise, (n, d, d))).astype(np.float32)
+    p.C = rng.normal(0, 2.0, (n, d, d)).astype(np.float32)
+    return p
+
+
+def to_state(p):
+    return autodiff.pack(*(torch.from_numpy(getattr(p, k)).cuda() for k in FIELDS))
+
+
+def to_oracle(p):
+    return O.State(x=p.x, v=p.v, F=p.F, C=p.C, Jp=p.Jp, mass=p.mass, volume=p.volume)
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("d,side,res", [(2, 24, 64), (3, 10, 32)])
+def test_one_step_vjp_matches_fp64_oracle_adjoint(d, side, res):
+    p = jelly_particles(d, side, seed=d)
+    n = p.n
+    rng = np.random.default_rng(11)
+    cot = {k: rng.normal(size=getattr(p, k).shape) for k in FIELDS}
+    print(f"[adjoint] one-step VJP d={d} n={n} res={res}", flush=True)
+    with MPMSimulation(p, MaterialModel.kJelly, res=res, E=1e4) as sim:
+        s0 = to_state(p).requires_grad_(True)
+        E = torch.tensor([1e4], dtype=torch.float32)
+        E.requires_grad_(True)
+        out = autodiff.rollout(sim, s0, E, steps=1, checkpoint_every=1)
+        go = autodiff.pack(*(torch.from_numpy(cot[k]).float().cuda() for k in FIELDS))
+        (out * go).sum().backward()
+        g = autodiff.unpack(s0.grad.detach(), d)
+        gE = float(E.grad)
+    # the cotangent the GPU saw is the FP32 rounding of `cot`
+    G = O.Grads(*(cot[k].astype(np.float32).astype(np.float64) for k in FIELDS))
+    ref = O.adjoint_jelly(to_oracle(p), G, res=res, E=1e4)
+    for k, gk in zip(FIELDS, g):
+        e = rel_err(gk.cpu().numpy().reshape(getattr(ref, k).shape), getattr(ref, k))
+        print(f"[adjoint] d={d} d{k}: normwise rel err vs FP64 adjoint {e:.2e}", flush=True)
+        assert e < 1e-4, (k, e)
+    assert abs(gE - ref.E) <= 1e-4 * abs(ref.E), (gE, ref.E)
+
+
+def test_backward_step_c_abi_and_permutations():
+    """mpm_backward_step on a state whose slot order is not the particle order."""
+    d = 2
+    p = jelly_particles(d, 20, seed=5)
+    n, ng = p.n, autodiff.channels(d)
+    rng = np.random.default_rng(3)
+    with MPMSimulation(p, MaterialModel.kJelly, res=64) as sim:
+        sim.advance(3)  # slots are now sorted by tile, not in particle order
+        mid = sim.particles().copy()
+        L, h = lib(), sim._handle()
+        snap = torch.empty(int(L.mpm_state_bytes(h)), dtype=torch.uint8, device="cuda")
+        check(L.mpm_save_state(h, snap.data_ptr()))
+        sim.advance(1)
+        gu = torch.from_numpy(rng.normal(size=(ng, n)).astype(np.float32)).cuda()
+        gslot = torch.empty_like(gu)
+        check(L.mpm_grad_from_particle_order(h, gu.data_ptr(), gslot.data_ptr()))
+        check(L.mpm_load_state(h, snap.data_ptr()))
+        gin = torch.empty_like(gu)
+        gE = torch.zeros(1, dtype=torch.float64, device="cuda")
+        check(L.mpm_backward_step(h, gslot.data_ptr(), gin.data_ptr(), gE.data_ptr()))
+        g0 = torch.empty_like(gu)
+        check(L.mpm_grad_to_particle_order(h, gin.data_ptr(), g0.data_ptr()))
+        check(L.mpm_synchronize(h))
+        assert sim.steps_taken == 4
+        # the state was left at the snapshot
+        np.testing.assert_array_equal(sim.particles().x, mid.x)
+    gun = gu.cpu().numpy().astype(np.float64)
+    G = O.Grads(*(np.ascontiguousarray(gun[s].T).reshape(getattr(mid, k).shape) for k, s in
+                  zip(FIELDS, (slice(0, 2), slice(2, 4), slice(4, 8), slice(8, 12)))))
+    ref = O.adjoint_jelly(to_oracle(mid), G, res=64)
+    got = g0.cpu().numpy()
+    for k, s in zip(FIELDS, (slice(0, 2), slice(2, 4), slice(4, 8), slice(8, 12))):
+        e = rel_err(got[s].T.reshape(getattr(ref, k).shape), getattr(ref, k))
+        assert e < 1e-4, (k, e)
+    assert abs(float(gE) - ref.E) <= 1e-4 * abs(ref.E)
+
+
+def com_loss(out, d, target, n_per_scene):
+    x = autodiff.unpack(out, d)[0]
+    loss = 0.0
+    start = 0
+    for s, m in enumerate(n_per_scene):
+        com = x[start:start + m].double().mean(0)
+        loss = loss + ((com - target[s]) ** 2).sum()
+        start += m
+    return loss
+
+
+def affine_jelly(d, side, lo, hi, seed):
+    """A cube with an affine initial velocity field plus noise, so that it shears, rotates and
+    deforms and the elastic forces (and with them dL/dE) matter."""
+    x = scenes.cube(d, side, lo, hi)
+    rng = np.random.default_rng(seed)
+    p = Particles.from_positions(x)
+    p.v = ((x - x.mean(0)) @ rng.normal(0, 2.0, (d, d)).T + rng.normal(0, 0.1, x.shape)).astype(np.float32)
+    return p
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_multi_step_gradient_matches_fp64_adjoint_and_finite_differences(d):
+    """GPU gradients of L = <c, x_T> + <c', v_T> + <c'', F_T> after `steps` steps (checkpointed)
+    against the FP64 oracle's chained adjoint on every field, and against central differences of
+    the FP64 oracle rollout with eps = 1e-3 for the initial velocities and E (relative)."""
+    side, res, steps, every = (20, 128, 64, 16) if d == 2 else (8, 64, 32, 8)
+    E0 = 5e3
+    p = affine_jelly(d, side, 0.3, 0.5, seed=d)
+    n = p.n
+    rng = np.random.default_rng(7)
+    G = O.Grads(x=rng.normal(size=(n, d)), v=rng.normal(size=(n, d)), F=rng.normal(size=(n, d, d)),
+                C=np.zeros((n, d, d)))
+    print(f"[adjoint] multi-step d={d} n={n} steps={steps} k={every}", flush=True)
+    with MPMSimulation(p, MaterialModel.kJelly, res=res, E=E0) as sim:
+        s0 = to_state(p).requires_grad_(True)
+        E = torch.tensor([E0], dtype=torch.float64, requires_grad=True)
+        out = autodiff.rollout(sim, s0, E, steps, every)
+        w = autodiff.pack(*(torch.from_numpy(getattr(G, k)).float().cuda() for k in FIELDS))
+        (out * w).sum().backward()
+        g = [t.detach().cpu().numpy().astype(np.float64) for t in autodiff.unpack(s0.grad, d)]
+        gE = float(E.grad)
+    G32 = O.Grads(*(getattr(G, k).astype(np.float32).astype(np.float64) for k in FIELDS))
+    ref = O.rollout_vjp_jelly(to_oracle(p), G32, steps, res=res, E=E0)
+    for k, gk in zip(FIELDS, g):
+        e = rel_err(gk.reshape(getattr(ref, k).shape), getattr(ref, k))
+        print(f"[adjoint] d={d} d{k}: normwise rel err vs FP64 chained adjoint {e:.2e}", flush=True)
+        assert e < 1e-3, (k, e)
+    s64 = to_oracle(p).astype(np.float64)
+
+    def L(st, E=E0):
+        o = O.run(st, O.JELLY, res=res, E=E, steps=steps, fp64=True)
+        return sum(float(np.sum(getattr(G32, k) * getattr(o, k))) for k in ("x", "v", "F"))
+
+    eps = 1e-3
+    for trial in range(2):
+        u = rng.normal(size=(n, d))
+        sp, sm = s64.copy(), s64.copy()
+        sp.v = s64.v + eps * u
+        sm.v = s64.v - eps * u
+        fd = (L(sp) - L(sm)) / (2 * eps)
+        an = float(np.sum(g[1] * u))
+        print(f"[adjoint] d={d} v0 direction {trial}: fd={fd:.6e} gpu={an:.6e}", flush=True)
+        assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
+    hE = eps * E0
+    fdE = (L(s64, E0 + hE) - L(s64, E0 - hE)) / (2 * hE)
+    print(f"[adjoint] d={d} E: fd={fdE:.6e} gpu={gE:.6e}", flush=True)
+    assert abs(fdE - gE) <= 1e-3 * abs(fdE), (fdE, gE)
+
+
+def test_centre_of_mass_loss_gradient():
+    """The differentiable-jelly loss |com(x_T) - target|^2 on a falling cube: APIC transfers
+    conserve momentum and no particle reaches the boundary band, so com(x_T) = com(x_0) +
+    T dt (mean v_0 + g), every initial velocity gets the gradient 2 (com - target) T dt / n and
+    E none; checked on the GPU gradient."""
+    d, steps, every = 2, 128, 32
+    p = Particles.from_positions(scenes.cube(d, 60, 0.3, 0.5))
+    target = torch.tensor([[0.7, 0.3]], dtype=torch.float64, device="cuda")
+    with MPMSimulation(p, MaterialModel.kJelly, res=128, E=5e3) as sim:
+        s0 = to_state(p).requires_grad_(True)
+        E = torch.tensor([5e3], dtype=torch.float64, requires_grad=True)
+        out = autodiff.rollout(sim, s0, E, steps, every)
+        loss = com_loss(out, d, target, [p.n])
+        loss.backward()
+        com = autodiff.unpack(out.detach(), d)[0].double().mean(0)
+        expect = (2 * (com - target[0]) * steps * 1e-4 / p.n).cpu().numpy()
+        gv = autodiff.unpack(s0.grad, d)[1].cpu().numpy()
+        gx = autodiff.unpack(s0.grad, d)[0].cpu().numpy()
+    assert np.abs(gv - expect).max() <= 1e-3 * np.abs(expect).max(), (gv[:3], expect)
+    np.testing.assert_allclose(gx, np.broadcast_to(2 * (com - target[0]).cpu().numpy() / p.n, gx.shape),
+                               rtol=1e-3, atol=1e-3 * float(np.abs(gx).max()))
+    # the cube falls without deforming (F = I, no stress), so E only enters through rounding
+    assert abs(float(E.grad)) * 5e3 <= 1e-3 * float(loss), (float(E.grad), float(loss))
+
+
+def test_checkpoint_interval_does_not_change_gradients():
+    p = Particles.from_positions(scenes.cube(2, 30, 0.3, 0.5))
+    target = torch.tensor([[0.7, 0.3]], dtype=torch.float64, device="cuda")
+    grads = []
+    for every in (1, 7, 40):
+        with MPMSimulation(p, MaterialModel.kJelly, res=128, E=5e3) as sim:
+            s0 = to_state(p).requires_grad_(True)
+            E = torch.tensor([5e3], dtype=torch.float32, requires_grad=True)
+            com_loss(autodiff.rollout(sim, s0, E, 40, every), 2, target, [p.n]).backward()
+            grads.append((s0.grad.cpu().numpy(), float(E.grad)))
+    for g, e in grads[1:]:
+        np.testing.assert_array_equal(g, grads[0][0])
+        assert e == grads[0][1]
+
+
+def test_batched_gradients_equal_single_scene_gradients():
+    specs = [SceneSpec(s.particles, MaterialModel.kJelly, 48, E=s.E)
+             for s in scenes.jelly_3d_batched(seed=4, scenes=3, side_res=10)]
+    nps = [s.particles.n for s in specs]
+    rng = np.random.default_rng(1)
+    targets = torch.from_numpy(rng.uniform(0.2, 0.8, (len(specs), 3))).cuda()
+    steps, every = 24, 8
+    with MPMBatch(specs) as b:
+        s0 = torch.cat([to_state(s.particles) for s in specs], dim=1).requires_grad_(True)
+        E = torch.tensor([s.E for s in specs], dtype=torch.float64, requires_grad=True)
+        com_loss(autodiff.rollout(b, s0, E, steps, every), 3, targets, nps).backward()
+        g_batch = s0.grad.cpu().numpy()
+        gE_batch = E.grad.numpy()
+    start = 0
+    for s, sp in enumerate(specs):
+        with MPMSimulation(sp.particles, MaterialModel.kJelly, res=48, E=sp.E) as sim:
+            s1 = to_state(sp.particles).requires_grad_(True)
+            E1 = torch.tensor([sp.E], dtype=torch.float64, requires_grad=True)
+            com_loss(autodiff.rollout(sim, s1, E1, steps, every), 3, targets[s:s + 1], [sp.particles.n]).backward()
+            np.testing.assert_array_equal(g_batch[:, start:start + nps[s]], s1.grad.cpu().numpy())
+            assert gE_batch[s] == float(E1.grad), (s, gE_batch[s], float(E1.grad))
+        start += nps[s]
+
+
+def test_non_jelly_scenes_are_rejected():
+    p = Particles.from_positions(scenes.cube(2, 10, 0.4, 0.6))
+    with MPMSimulation(p, MaterialModel.kSnow, res=64) as sim:
+        s0 = to_state(p).requires_grad_(True)
+        E = torch.tensor([1e4], requires_grad=True)
+        out = autodiff.rollout(sim, s0, E, 2, 2)
+        with pytest.raises(NotImplementedError):
+            out.sum().backward()

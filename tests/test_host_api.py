@@ -36,6 +36,8 @@ def test_version_and_default_options_without_gpu():
     lib.mpm_default_options(ctypes.byref(o))
     assert o.identity_mode == 0 and o.device == 0
     assert ctypes.sizeof(mpm_options) == 16 * 4
+    from nuclearmpm_amd._lib import mpm_scene
+    assert ctypes.sizeof(mpm_scene) == 32
 
 
 def test_null_handle_errors_are_reported_without_gpu():
@@ -44,6 +46,33 @@ def test_null_handle_errors_are_reported_without_gpu():
     assert b"NULL" in lib.mpm_last_error()
     assert lib.mpm_num_particles(None) == -1
     assert lib.mpm_destroy(None) == 0
+    assert lib.mpm_state_bytes(None) == -1
+    for name in ("mpm_save_state", "mpm_load_state", "mpm_get_state_device", "mpm_set_state_device"):
+        assert getattr(lib, name)(None, None) == 1
+    assert lib.mpm_backward_step(None, None, None, None) == 1
+    assert lib.mpm_backward_segment(None, 1, None, None, None) == 1
+    assert lib.mpm_set_scene_elasticity(None, 0, 1e4, 0.2) == 1
+
+
+def test_state_channels_and_packing():
+    import torch
+
+    from nuclearmpm_amd import autodiff
+
+    lib = nm.lib()
+    assert [lib.mpm_grad_channels(d) for d in (1, 2, 3, 4)] == [-1, 12, 24, -1]
+    assert autodiff.channels(2) == 12 and autodiff.channels(3) == 24
+    n, d = 5, 3
+    g = torch.Generator().manual_seed(0)
+    x, v = torch.randn(n, d, generator=g), torch.randn(n, d, generator=g)
+    F, C = torch.randn(n, d, d, generator=g), torch.randn(n, d, d, generator=g)
+    s = autodiff.pack(x, v, F, C)
+    assert s.shape == (24, n) and s.is_contiguous()
+    # channel layout: x, v, F row-major, C row-major
+    assert torch.equal(s[0:3].t(), x) and torch.equal(s[3:6].t(), v)
+    assert torch.equal(s[6 + 1, 2], F[2, 0, 1]) and torch.equal(s[15 + 5, 4], C[4, 1, 2])
+    for a, b in zip(autodiff.unpack(s, d), (x, v, F, C)):
+        assert torch.equal(a, b)
 
 
 def test_create_rejects_bad_arguments_before_touching_the_device():
