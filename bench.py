@@ -176,6 +176,10 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and forward+backward legs")
+    ap.add_argument("--no-graph", action="store_true", help="launch the step kernels one by one (no hipGraph)")
+    ap.add_argument("--fast-svd", action="store_true", help="engine option fast_svd (not bitwise the reference)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 5 device copies of 256 MiB (the reference kernel for HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path in this build "
@@ -189,7 +193,18 @@ def main() -> int:
     n = p.n
     dim, mat = sc.particles.dim, int(sc.material)
 
-    sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity)
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity,
+                        graph=not args.no_graph, fast_svd=args.fast_svd)
     sim.advance(args.warmup)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -248,6 +263,8 @@ def main() -> int:
             "dim": dim,
             "material": "snow",
             "mode": "fwd",
+            "hipgraph": not args.no_graph,
+            "svd": "fast (not bitwise the reference)" if args.fast_svd else "nclr_svd order (bitwise the reference's FP32)",
         },
         "roofline": {
             "kernel": dom,

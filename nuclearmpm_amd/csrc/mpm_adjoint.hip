@@ -217,14 +217,14 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
 #pragma unroll
         for (int c = 0; c < D; ++c) s_bound[threadIdx.x >> 6][c] = bound[c];
     __syncthreads();
-    float scale[D], inv_scale[D];
+    float scale[D];
+    double inv_scale[D];
 #pragma unroll
     for (int c = 0; c < D; ++c)
         fixed_point_scale((s_bound[0][c] + s_bound[1][c]) + (s_bound[2][c] + s_bound[3][c]), scale[c], inv_scale[c]);
 
     // pass 2: gather the updated grid (for the forward C and the weight derivatives), scatter the
     // quantised grid cotangent, write the partial x and F cotangents
-    constexpr float kLimit = 4.6116860e18f;  // 2^62
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
                 const float val = weight * (vb[k][a] + four_inv_dx * cd);
                 wbar += gvel[a] * (vb[k][a] + four_inv_dx * cd);
                 const float y = val * scale[a];
-                if (fabsf(y) < kLimit) atomicAdd(&acc[a * REG + r], (unsigned long long)(long long)llrintf(y));
+                if (fabsf(y) < kFixedLimit) atomicAdd(&acc[a * REG + r], fixed_point(y));
                 else atomicOr(&nonfinite[r], 1u << a);
             }
 #pragma unroll
@@ -319,10 +319,8 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
         float out[3] = {0.0f, 0.0f, 0.0f};
         const uint32_t bad = nonfinite[r];
 #pragma unroll
-        for (int c = 0; c < D; ++c) {
-            const long long q = (long long)acc[c * REG + r];
-            out[c] = (bad >> c) & 1u ? nan : float(double(q) * double(inv_scale[c]));
-        }
+        for (int c = 0; c < D; ++c)
+            out[c] = (bad >> c) & 1u ? nan : fixed_point_value(acc[c * REG + r], inv_scale[c]);
         dst[r] = make_float4(out[0], out[1], out[2], 0.0f);
     }
 }

@@ -157,14 +157,32 @@ NCLR_DI void item_tile(const uint32_t* __restrict__ item_offset, uint32_t ntiles
     chunk = item - item_offset[lo];
 }
 
-// Per-channel fixed-point scale 2^e for a workgroup whose summed |contribution| bound is `total`:
-// total * 2^e <= 2^61, so the 64-bit integer sum of the quantised values cannot overflow.
-NCLR_DI void fixed_point_scale(float total, float& scale, float& inv_scale) {
+// Exact LDS reductions in fixed point.  A workgroup bounds the sum of |contributions| of every
+// channel (`total`) and scales the channel by 2^e with total * 2^e <= 2^61: each quantised
+// contribution is below 2^62 in magnitude and the 64-bit integer sum is exact.  The quantum,
+// 2^-61 of the bound, stays below the FP32 rounding of even the small sums of a region's edge
+// nodes (a coarser quantum, 2^-50, measurably moves ill-conditioned 3D snow particles).
+constexpr float kFixedLimit = 4.6116860e18f;  // 2^62
+
+NCLR_DI void fixed_point_scale(float total, float& scale, double& inv_scale) {
     int ex = 0;
     (void)frexpf(total > 0.0f && total < 3.0e38f ? total : 1.0f, &ex);  // total < 2^ex
     const int e = min(127, 61 - ex);                                      // in [-68, 127]
     scale = ldexpf(1.0f, e);
-    inv_scale = ldexpf(1.0f, -e);
+    inv_scale = ldexp(1.0, -e);
+}
+
+// round-to-nearest-even integer of y (|y| < 2^62) as a two's complement 64-bit value, in double:
+// r = rint(y) exactly, r = hi 2^32 + lo with lo in [0, 2^32), both halves converted directly
+NCLR_DI unsigned long long fixed_point(float y) {
+    const double r = rint(double(y));
+    const double hi = floor(r * 2.3283064365386963e-10);  // 2^-32
+    const double lo = fma(hi, -4294967296.0, r);           // exact
+    return (unsigned long long)(uint32_t)lo | ((unsigned long long)(uint32_t)(int32_t)hi << 32);
+}
+
+NCLR_DI float fixed_point_value(unsigned long long q, double inv_scale) {
+    return float(double((long long)q) * inv_scale);
 }
 
 }  // namespace nclr_dev

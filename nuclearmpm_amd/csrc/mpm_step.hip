@@ -109,6 +109,68 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ sta
 // ---------------------------------------------------------------------------------------------
 // P2G (src/nclr.h:104-165)
 // ---------------------------------------------------------------------------------------------
+
+// One particle's 3^D contributions weight * (m v + A dpos) (momentum channels 0..D-1) and
+// weight * m (channel 3), quantised into the LDS accumulators of its region (offset rb).  The
+// products A[c][b] dpos_b of the outer axes are formed once per outer offset and those of the last
+// axis once per particle, so every value keeps the operation order of
+// weight * (m v_c + ((A_c0 dpos_0 + A_c1 dpos_1) + A_c2 dpos_2)) with weight = (w_0 w_1) w_2.
+// CHECKED: contributions at or beyond the fixed-point range flag their node channel instead.
+template <int D, bool CHECKED>
+NCLR_DI void p2g_deposit(unsigned long long* __restrict__ acc, uint32_t* __restrict__ nonfinite, int rb,
+                         const float (&fx)[D], const M<D>& A, const float (&mv)[D], float m, float dx,
+                         const float (&scale)[4]) {
+    using TL = Tile<D>;
+    constexpr int REG = TL::REGION;
+    float w[D][3], dp[D][3], last[D][3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        bspline_weights(fx[a], w[a]);
+#pragma unroll
+        for (int o = 0; o < 3; ++o) dp[a][o] = (float(o) - fx[a]) * dx;
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) last[c][o] = A.a[c][D - 1] * dp[D - 1][o];
+    auto node = [&](int ro, float weight, const float (&outer)[D], int ol) {
+        float val[4];
+#pragma unroll
+        for (int c = 0; c < D; ++c) val[c] = weight * (mv[c] + (outer[c] + last[c][ol]));
+        val[3] = weight * m;
+        uint32_t bad = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (D == 2 && c == 2) continue;
+            const float y = val[c] * scale[c];
+            if (!CHECKED || fabsf(y) < kFixedLimit) atomicAdd(&acc[c * REG + rb + ro], fixed_point(y));
+            else bad |= 1u << c;
+        }
+        if (CHECKED && bad) atomicOr(&nonfinite[rb + ro], bad);
+    };
+    if constexpr (D == 3) {
+#pragma unroll
+        for (int o0 = 0; o0 < 3; ++o0)
+#pragma unroll
+            for (int o1 = 0; o1 < 3; ++o1) {
+                float outer[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) outer[c] = A.a[c][0] * dp[0][o0] + A.a[c][1] * dp[1][o1];
+                const float w01 = w[0][o0] * w[1][o1];
+#pragma unroll
+                for (int o2 = 0; o2 < 3; ++o2) node((o0 * TL::R + o1) * TL::R + o2, w01 * w[2][o2], outer, o2);
+            }
+    } else {
+#pragma unroll
+        for (int o0 = 0; o0 < 3; ++o0) {
+            float outer[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) outer[c] = A.a[c][0] * dp[0][o0];
+#pragma unroll
+            for (int o1 = 0; o1 < 3; ++o1) node(o0 * TL::R + o1, w[0][o0] * w[1][o1], outer, o1);
+        }
+    }
+}
 template <int D, int MAT, int PPT, bool EXACT>
 __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, const uint32_t* __restrict__ perm,
                                                   const uint32_t* __restrict__ tile_start,
@@ -143,7 +205,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
 
     // pass 1: per particle the fused affine matrix, kept in registers, and a bound of the sum of
     // |contributions| per channel: sum_nodes |w| (|m v_c| + 2.5 dx sum_k |A_ck|)
-    bool valid[PPT];
+    bool valid[PPT], finite[PPT];
     int rb[PPT];
     float fxs[PPT][D], mvs[PPT][D], ms[PPT];
     M<D> As[PPT];
@@ -152,6 +214,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
     for (int k = 0; k < PPT; ++k) {
         const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
         valid[k] = false;
+        finite[k] = true;
         rb[k] = 0;
         ms[k] = 0.0f;
         if (j >= j1) continue;
@@ -193,12 +256,14 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
 #pragma unroll
             for (int b = 0; b < D; ++b) s += fabsf(As[k].a[c][b]);
             const float bc = wsum * (fabsf(mvs[k][c]) + 2.5f * sp.dx * s);
+            finite[k] = finite[k] && bc < 3.0e38f;
             if (bc < 3.0e38f) bound[c] += bc;  // non-finite contributions take the flag path
         }
         const float bm = wsum * fabsf(mass);
+        finite[k] = finite[k] && bm < 3.0e38f;
         if (bm < 3.0e38f) bound[3] += bm;
     }
-    // workgroup sum of the bounds -> per channel scale 2^e with bound * 2^e <= 2^61
+    // workgroup sum of the bounds -> per channel scale 2^e with bound * 2^e <= 2^50
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         float b = bound[c];
@@ -210,58 +275,23 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
 #pragma unroll
         for (int c = 0; c < 4; ++c) s_bound[threadIdx.x >> 6][c] = bound[c];
     __syncthreads();
-    float scale[4], inv_scale[4];
+    float scale[4];
+    double inv_scale[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const float total = (s_bound[0][c] + s_bound[1][c]) + (s_bound[2][c] + s_bound[3][c]);
         fixed_point_scale(total, scale[c], inv_scale[c]);
     }
 
-    // pass 2: quantised contributions -> 64-bit integer LDS atomics
-    constexpr int K = D == 2 ? 9 : 27;
-    constexpr float kLimit = 4.6116860e18f;  // 2^62
+    // pass 2: quantised contributions -> 64-bit integer LDS atomics.  A particle whose bound is
+    // finite has every |contribution * scale| below 2^51 and needs no per-contribution check.
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
-        float w[D][3];
-#pragma unroll
-        for (int a = 0; a < D; ++a) bspline_weights(fxs[k][a], w[a]);
-#pragma unroll
-        for (int o = 0; o < K; ++o) {
-            int off[D];
-            offsets<D>(o, off);
-            float dpos[D];
-            int ro = 0;
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                dpos[a] = (float(off[a]) - fxs[k][a]) * sp.dx;
-                ro = ro * TL::R + off[a];
-            }
-            float weight = w[0][off[0]];
-#pragma unroll
-            for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
-            const int r = rb[k] + ro;
-            float val[4];
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                float ad = As[k].a[c][0] * dpos[0];
-#pragma unroll
-                for (int b = 1; b < D; ++b) ad = ad + As[k].a[c][b] * dpos[b];
-                val[c] = weight * (mvs[k][c] + ad);
-            }
-            if constexpr (D == 2) val[2] = 0.0f;
-            val[3] = weight * ms[k];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if (D == 2 && c == 2) continue;
-                const float y = val[c] * scale[c];
-                if (fabsf(y) < kLimit) {
-                    atomicAdd(&acc[c * REG + r], (unsigned long long)(long long)llrintf(y));
-                } else {
-                    atomicOr(&nonfinite[r], 1u << c);
-                }
-            }
-        }
+        if (finite[k])
+            p2g_deposit<D, false>(acc, nonfinite, rb[k], fxs[k], As[k], mvs[k], ms[k], sp.dx, scale);
+        else
+            p2g_deposit<D, true>(acc, nonfinite, rb[k], fxs[k], As[k], mvs[k], ms[k], sp.dx, scale);
     }
     __syncthreads();
     // every work item owns a slot: plain stores, no global atomics
@@ -271,10 +301,8 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
         float out[4];
         const uint32_t bad = nonfinite[r];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const long long q = (long long)acc[c * REG + r];
-            out[c] = (bad >> c) & 1u ? nan : float(double(q) * double(inv_scale[c]));
-        }
+        for (int c = 0; c < 4; ++c)
+            out[c] = (bad >> c) & 1u ? nan : fixed_point_value(acc[c * REG + r], inv_scale[c]);
         dst[r] = make_float4(out[0], out[1], out[2], out[3]);
     }
 }
