@@ -144,6 +144,17 @@ NCLR_DI void tile_coords(uint32_t t, int n, int (&c)[D]) {
     }
 }
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin by workgroup id.
+// Launches that read neighbouring data from neighbouring blocks round their grid up to a multiple
+// of 8 and map workgroup b to logical block (b % 8) * (grid / 8) + b / 8, so that consecutive
+// logical blocks share an XCD and its L2.  Logical blocks at or beyond the real count exit.
+constexpr uint32_t kXcds = 8;
+NCLR_DI uint32_t xcd_block() {
+    const uint32_t per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+inline unsigned xcd_grid(int64_t blocks) { return unsigned((blocks + kXcds - 1) / kXcds * kXcds); }
+
 // Work item -> (tile, chunk within the tile) by binary search over the exclusive item scan.
 NCLR_DI void item_tile(const uint32_t* __restrict__ item_offset, uint32_t ntiles, uint32_t item, uint32_t& tile,
                        uint32_t& chunk) {

@@ -45,15 +45,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
                                                              int64_t ntiles) {
     __shared__ uint32_t counts[256];
     for (int i = threadIdx.x; i < 256; i += kSortThreads) counts[i] = 0;
-    __syncthreads();
     const int64_t base = int64_t(blockIdx.x) * kTileKeys;
+    // all loads first, so that their latencies overlap
+    uint32_t d[kSortItems];
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const int64_t idx = base + int64_t(r) * kSortThreads + threadIdx.x;
-        const bool valid = idx < n;
-        const uint32_t d = valid ? (keys[idx] >> shift) & dmask : 0u;
-        const uint64_t peers = match_digit(d, valid);
+        d[r] = idx < n ? (keys[idx] >> shift) & dmask : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const bool valid = base + int64_t(r) * kSortThreads + threadIdx.x < n;
+        const uint64_t peers = match_digit(d[r], valid);
         // the lowest lane of each group adds the group's size
-        if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&counts[d], uint32_t(__popcll(peers)));
+        if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&counts[d[r]], uint32_t(__popcll(peers)));
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 256; i += kSortThreads) hist[int64_t(i) * ntiles + blockIdx.x] = counts[i];
@@ -114,11 +120,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     for (int w = 0; w < kSortWaves; ++w) wave_count[w][t] = 0;
     __syncthreads();
     const int64_t base = int64_t(blockIdx.x) * kTileKeys;
+    // all loads first, so that their latencies overlap; then the ranking rounds in input order
+    uint32_t keys_r[kSortItems], vals_r[kSortItems];
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const int64_t idx = base + int64_t(r) * kSortThreads + t;
         const bool valid = idx < n;
-        const uint32_t key = valid ? keys_in[idx] : 0u;
-        const uint32_t val = valid ? (identity_values ? uint32_t(idx) : vals_in[idx]) : 0u;
+        keys_r[r] = valid ? keys_in[idx] : 0u;
+        vals_r[r] = valid ? (identity_values ? uint32_t(idx) : vals_in[idx]) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int64_t idx = base + int64_t(r) * kSortThreads + t;
+        const bool valid = idx < n;
+        const uint32_t key = keys_r[r];
+        const uint32_t val = vals_r[r];
         const uint32_t d = (key >> shift) & dmask;
         const uint64_t peers = match_digit(d, valid);
         const uint32_t rank = uint32_t(__popcll(peers & lanemask_lt()));

@@ -74,8 +74,8 @@ __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict_
 // (item_offset[ntiles] = total).  One workgroup of 1024 threads, coalesced rounds of 4096 tiles.
 __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                 uint32_t ntiles, uint32_t chunk, uint32_t* __restrict__ item_offset) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
+    __shared__ uint32_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     uint32_t carry = 0;
     for (uint32_t b = 0; b <= ntiles; b += 4096) {
         uint32_t v[4];
@@ -86,22 +86,33 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ sta
             v[k] = tile < ntiles ? (end[tile] - start[tile] + chunk - 1) / chunk : 0u;
             s += v[k];
         }
-        part[t] = s;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const uint32_t u = t >= off ? part[t - off] : 0u;
-            __syncthreads();
-            part[t] += u;
-            __syncthreads();
+        // inclusive scan of s: within each wave by shuffles, then over the 16 wave totals
+        uint32_t x = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
         }
-        uint32_t run = carry + part[t] - s;
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t ws = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const uint32_t y = __shfl_up(ws, off);
+                if (lane >= off) ws += y;
+            }
+            if (lane < 16) wsum[lane] = ws;
+        }
+        __syncthreads();
+        uint32_t run = carry + (wave ? wsum[wave - 1] : 0u) + x - s;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t tile = b + uint32_t(t) * 4 + k;
             if (tile <= ntiles) item_offset[tile] = run;
             run += v[k];
         }
-        carry += part[1023];
+        carry += wsum[15];
         __syncthreads();
     }
 }
@@ -320,7 +331,8 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
     __shared__ int s_first[NC];  // first P2G work item of candidate tile c (its slot), or -1
     __shared__ int s_count[NC];  // number of work items of candidate tile c
     __shared__ uint32_t s_active;
-    const uint32_t nt_idx = blockIdx.x;
+    const uint32_t nt_idx = xcd_block();
+    if (nt_idx >= p.node_tiles) return;
     const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_node_tile(p, nt_idx)];
     int nto[D];
     tile_coords<D>(nt_idx - sp.node_tile_base, sp.ntn, nto);
@@ -442,7 +454,7 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
                                                     uint32_t* __restrict__ pid_out, const uint32_t* __restrict__ perm,
                                                     StepParams p, const float4* __restrict__ gv) {
     using L = Layout<D>;
-    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t j = int64_t(xcd_block()) * blockDim.x + threadIdx.x;
     if (j >= p.n) return;
     const int64_t i = perm[j];
     const int64_t s = p.stride;
@@ -701,11 +713,11 @@ template <int D, int MAT>
 static hipError_t g2p_one(const LaunchArgs& a) {
     const int threads = a.g2p_threads > 0 ? a.g2p_threads : 256;
     if (D == 3 && !a.exact_svd) {
-        k_g2p_sorted<D, MAT, false><<<blocks_for(a.p.n, threads), threads, 0, a.stream>>>(
+        k_g2p_sorted<D, MAT, false><<<xcd_grid(blocks_for(a.p.n, threads)), threads, 0, a.stream>>>(
             a.state_in, a.state_out, a.pid_in, a.pid_out, a.perm, a.p, a.gv);
         return hipGetLastError();
     }
-    k_g2p_sorted<D, MAT, true><<<blocks_for(a.p.n, threads), threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in,
+    k_g2p_sorted<D, MAT, true><<<xcd_grid(blocks_for(a.p.n, threads)), threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in,
                                                                               a.pid_out, a.perm, a.p, a.gv);
     return hipGetLastError();
 }
@@ -729,9 +741,9 @@ hipError_t launch_grid(const LaunchArgs& a) {
     const unsigned tiles = a.p.node_tiles;
     if (tiles == 0) return hipSuccess;
     if (a.dim == 2)
-        k_grid_tiles<2><<<tiles, 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+        k_grid_tiles<2><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
     else
-        k_grid_tiles<3><<<tiles, 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+        k_grid_tiles<3><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
     return hipGetLastError();
 }
 
