@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
     __shared__ uint32_t s_tile, s_chunk;
     const uint32_t item = blockIdx.x;
     if (item >= item_offset[p.ntiles]) return;
-    if (threadIdx.x == 0) item_tile(item_offset, p.ntiles, item, s_tile, s_chunk);
+    if (threadIdx.x == 0) item_tile(p, item_offset, item, s_tile, s_chunk);
     for (int r = threadIdx.x; r < D * REG; r += blockDim.x) acc[r] = 0ull;
     for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
     __syncthreads();

@@ -94,6 +94,10 @@ struct nclr_mpm_sim {
     uint32_t* status = nullptr;
     uint32_t* active = nullptr;
     unsigned long long* active_sum = nullptr;
+    // incremental sort: valid while the state is stored in the order of the last sort and the
+    // tile bounds / work items of that sort are intact (cleared by uploads, loads, re-tuning)
+    uint32_t *uidx = nullptr, *ucnt = nullptr, *upref = nullptr, *ttotal = nullptr, *meta = nullptr;
+    bool sorted_valid = false;
     // adjoint buffers, allocated by the first backward call
     float4* slots_b = nullptr;
     float4* gbar = nullptr;
@@ -133,6 +137,11 @@ struct nclr_mpm_sim {
         a.stamps = stamps;
         a.status = status;
         a.active = active;
+        a.uidx = uidx;
+        a.ucnt = ucnt;
+        a.upref = upref;
+        a.ttotal = ttotal;
+        a.meta = meta;
         a.stream = stream;
         return a;
     }
@@ -196,7 +205,8 @@ struct nclr_mpm_sim {
     }
 
     int launch_step_kernels(const nclr_host::LaunchArgs& a) {
-        RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
+        if (sorted_valid) RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort_incremental(a); }));
+        else RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
         RC_TRY(timed(kP2G, [&] { return nclr_host::launch_p2g(a); }));
         RC_TRY(timed(kGrid, [&] { return nclr_host::launch_grid(a); }));
         RC_TRY(timed(kG2P, [&] { return nclr_host::launch_g2p(a); }));
@@ -214,7 +224,7 @@ struct nclr_mpm_sim {
     int capture(int parity) {
         const nclr_host::LaunchArgs a = args(parity);
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = nclr_host::launch_sort(a);
+        hipError_t e = nclr_host::launch_sort_incremental(a);
         if (e == hipSuccess) e = nclr_host::launch_p2g(a);
         if (e == hipSuccess) e = nclr_host::launch_grid(a);
         if (e == hipSuccess) e = nclr_host::launch_g2p(a);
@@ -231,7 +241,9 @@ struct nclr_mpm_sim {
     }
 
     int step() {
-        const bool use_graph = opts.graph && !opts.timing;
+        // graphs hold the incremental sort; a step after an upload or load runs eagerly with the
+        // full radix sort, which also rebuilds the tile bounds and work items
+        const bool use_graph = opts.graph && !opts.timing && sorted_valid;
         if (use_graph) {
             if (!graph_exec[cur]) RC_TRY(capture(cur));
             HIP_TRY(hipGraphLaunch(graph_exec[cur], stream));
@@ -242,6 +254,7 @@ struct nclr_mpm_sim {
             if (opts.timing) RC_TRY(record_stop(pe));
         }
         if (p.n > 0) cur ^= 1;
+        sorted_valid = true;
         steps += 1;
         return MPM_OK;
     }
@@ -259,6 +272,10 @@ struct nclr_mpm_sim {
                 return fail(MPM_ERR_OUT_OF_RANGE,
                             "out_of_range: a particle's interpolation stencil left the grid "
                             "(position not finite or outside [0, 1 - 1.5 dx) on some axis)");
+            if (st & nclr_dev::kStatusMovedTooFar)
+                return fail(MPM_ERR_OUT_OF_RANGE,
+                            "a particle moved more than one tile in one step (the state was not produced "
+                            "by the step; set it with mpm_set_particles)");
             return fail(MPM_ERR_HIP, "device status " + std::to_string(st));
         }
         return MPM_OK;
@@ -295,6 +312,7 @@ struct nclr_mpm_sim {
             if (volume) host[(fJ + 2) * stride + i] = volume[i];
         }
         cur = 0;
+        sorted_valid = false;
         HIP_TRY(hipMemcpyAsync(state[0], host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         HIP_TRY(nclr_host::launch_iota_u32(pid[0], p.stride, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -319,6 +337,7 @@ struct nclr_mpm_sim {
             HIP_TRY(hipStreamSynchronize(stream));
             drop_graphs();
         }
+        if (chunk != p.chunk) sorted_valid = false;  // the stored work items use the old chunk
         p.chunk = chunk;
         g2p_threads = threads;
         return MPM_OK;
@@ -344,6 +363,7 @@ struct nclr_mpm_sim {
         int rc = (e0 && e1) ? MPM_OK : fail(MPM_ERR_HIP, "hipEventCreate failed");
         auto restore = [&]() -> int {
             cur = cur0;
+            sorted_valid = false;
             HIP_TRY(hipMemcpyAsync(state[cur], saved, state_bytes, hipMemcpyDeviceToDevice, stream));
             HIP_TRY(hipMemcpyAsync(pid[cur], saved_pid, pid_bytes, hipMemcpyDeviceToDevice, stream));
             HIP_TRY(hipStreamSynchronize(stream));
@@ -404,6 +424,7 @@ struct nclr_mpm_sim {
     }
     int load_state(const void* src) {
         const char* s = static_cast<const char*>(src);
+        sorted_valid = false;
         HIP_TRY(hipMemcpyAsync(state[cur], s, state_floats() * sizeof(float), hipMemcpyDeviceToDevice, stream));
         HIP_TRY(hipMemcpyAsync(pid[cur], s + state_floats() * sizeof(float), size_t(p.stride) * sizeof(uint32_t),
                                hipMemcpyDeviceToDevice, stream));
@@ -510,9 +531,11 @@ struct nclr_mpm_sim {
         void* bufs[] = {state[0], state[1], pid[0],  pid[1],   keys,    perm,      keys_tmp, perm_tmp,
                         hist,     tile_start, tile_end, item_offset, slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, gtmp[0],
-                        gtmp[1],  tape};
+                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, p.item_tiles};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
+        uidx = ucnt = upref = ttotal = meta = nullptr;
+        p.item_tiles = nullptr;
         slots_b = gbar = nullptr;
         epart = gtmp[0] = gtmp[1] = nullptr;
         tape = nullptr;
@@ -622,6 +645,8 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
     const size_t gv_nodes = size_t(node_tiles) * nclr_host::tile_nodes(dim);
     const size_t slot_nodes = size_t(nclr_host::max_items(n, uint32_t(tiles))) * nclr_host::region_nodes(dim);
     const size_t nn = size_t(std::max<int64_t>(n, 1));
+    const size_t bucket_cells =
+        size_t(nclr_host::max_items(n, uint32_t(tiles))) * size_t(nclr_host::bucket_offsets(dim));
     if (dmalloc(&h->state[0], state_floats) != hipSuccess || dmalloc(&h->state[1], state_floats) != hipSuccess ||
         dmalloc(&h->pid[0], size_t(p.stride)) != hipSuccess || dmalloc(&h->pid[1], size_t(p.stride)) != hipSuccess ||
         dmalloc(&h->keys, nn) != hipSuccess || dmalloc(&h->perm, nn) != hipSuccess ||
@@ -632,7 +657,10 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
         dmalloc(&h->gv, gv_nodes) != hipSuccess || dmalloc(&h->stamps, size_t(node_tiles)) != hipSuccess ||
         dmalloc(&h->status, 1) != hipSuccess || dmalloc(&h->active, size_t(node_tiles)) != hipSuccess ||
         dmalloc(&h->active_sum, 1) != hipSuccess || dmalloc(&h->stamp, 1) != hipSuccess ||
-        dmalloc(&h->d_scenes, size_t(nscenes)) != hipSuccess)
+        dmalloc(&h->d_scenes, size_t(nscenes)) != hipSuccess || dmalloc(&h->uidx, nn) != hipSuccess ||
+        dmalloc(&h->ucnt, bucket_cells) != hipSuccess || dmalloc(&h->upref, bucket_cells) != hipSuccess ||
+        dmalloc(&h->ttotal, nbins) != hipSuccess || dmalloc(&h->meta, 4) != hipSuccess ||
+        dmalloc(&h->p.item_tiles, size_t(nclr_host::max_items(n, uint32_t(tiles)))) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
     p.scenes = h->d_scenes;
     p.stamp = h->stamp;
@@ -962,6 +990,7 @@ int mpm_set_state_device(mpm_handle h, const float* src) {
                              size_t(n) * sizeof(float), size_t(ng), hipMemcpyDeviceToDevice, h->stream));
     HIP_TRY(nclr_host::launch_iota_u32(h->pid[o], stride, h->stream));
     h->cur = o;
+    h->sorted_valid = false;
     return MPM_OK;
 }
 

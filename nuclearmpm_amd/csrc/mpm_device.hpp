@@ -155,17 +155,16 @@ NCLR_DI uint32_t xcd_block() {
 }
 inline unsigned xcd_grid(int64_t blocks) { return unsigned((blocks + kXcds - 1) / kXcds * kXcds); }
 
-// Work item -> (tile, chunk within the tile) by binary search over the exclusive item scan.
-NCLR_DI void item_tile(const uint32_t* __restrict__ item_offset, uint32_t ntiles, uint32_t item, uint32_t& tile,
+// Work item -> (tile, chunk within the tile), from the item table written with the item scan.
+NCLR_DI void item_tile(const StepParams& p, const uint32_t* __restrict__ item_offset, uint32_t item, uint32_t& tile,
                        uint32_t& chunk) {
-    uint32_t lo = 0, hi = ntiles;  // item_offset[lo] <= item < item_offset[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (item_offset[mid] <= item) lo = mid;
-        else hi = mid;
-    }
-    tile = lo;
-    chunk = item - item_offset[lo];
+    tile = p.item_tiles[item];
+    chunk = item - item_offset[tile];
+}
+
+// Records the tile of items [first, first + count).
+NCLR_DI void write_item_tiles(uint32_t* __restrict__ item_tiles, uint32_t first, uint32_t count, uint32_t tile) {
+    for (uint32_t c = 0; c < count; ++c) item_tiles[first + c] = tile;
 }
 
 // Exact LDS reductions in fixed point.  A workgroup bounds the sum of |contributions| of every

@@ -10,6 +10,7 @@
 namespace nclr_dev {
 
 constexpr uint32_t kStatusOutOfRange = 1u;
+constexpr uint32_t kStatusMovedTooFar = 2u;  // a particle crossed more than one tile in a step
 constexpr int kMixed = 3;  // material template value: per-scene material chosen at run time
 
 // Particle tiles: S^D cells (S = 16 in 2D, 8 in 3D).  A tile's P2G region covers the nodes its
@@ -63,6 +64,7 @@ struct StepParams {
     int64_t stride;       // floats between consecutive fields of the SoA state
     const SceneParams* scenes;
     const uint32_t* stamp;  // device word: stamp of the current step (incremented by the key kernel)
+    uint32_t* item_tiles;   // max_items: tile of each P2G work item (written with item_offset)
 };
 
 }  // namespace nclr_dev
@@ -93,6 +95,12 @@ struct LaunchArgs {
     uint32_t* stamps;          // node tiles
     uint32_t* status;
     uint32_t* active;          // node tiles: nodes with post-P2G mass > 0
+    // incremental sort (mpm_bucket.hip)
+    uint32_t* uidx;            // n: old work item of each slot
+    uint32_t* ucnt;            // max_items * offsets: per old item, particles per neighbour offset
+    uint32_t* upref;           // max_items * offsets: per (old item, offset), earlier particles bound alike
+    uint32_t* ttotal;          // ntiles: particles per new tile
+    uint32_t* meta;            // 4 counters
     hipStream_t stream;
 };
 
@@ -115,6 +123,10 @@ hipError_t launch_permute_channels(const uint32_t* pid, const float* src, int64_
                                    int64_t dst_stride, int channels, int64_t n, bool to_particle_order,
                                    hipStream_t s);
 hipError_t launch_sort(const LaunchArgs& a);
+// The stable sort by tile for a state that is stored in the order of the previous sort, whose
+// tile bounds and work items are still in tile_start / tile_end / item_offset (same chunk size).
+hipError_t launch_sort_incremental(const LaunchArgs& a);
+int bucket_offsets(int dim);
 hipError_t launch_p2g(const LaunchArgs& a);
 hipError_t launch_grid(const LaunchArgs& a);
 hipError_t launch_g2p(const LaunchArgs& a);

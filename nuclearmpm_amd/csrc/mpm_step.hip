@@ -73,18 +73,23 @@ __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict_
 // P2G work items: ceil(count / chunk) per tile, exclusive scan into item_offset[0..ntiles]
 // (item_offset[ntiles] = total).  One workgroup of 1024 threads, coalesced rounds of 4096 tiles.
 __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                                                uint32_t ntiles, uint32_t chunk, uint32_t* __restrict__ item_offset) {
+                                                uint32_t ntiles, uint32_t chunk, uint32_t* __restrict__ item_offset,
+                                                uint32_t* __restrict__ item_tiles, uint32_t* __restrict__ meta) {
     __shared__ uint32_t wsum[16];
+    __shared__ uint32_t s_placed;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    uint32_t carry = 0;
+    if (t == 0) s_placed = 0;
+    uint32_t carry = 0, placed = 0;
     for (uint32_t b = 0; b <= ntiles; b += 4096) {
         uint32_t v[4];
         uint32_t s = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t tile = b + uint32_t(t) * 4 + k;
-            v[k] = tile < ntiles ? (end[tile] - start[tile] + chunk - 1) / chunk : 0u;
+            const uint32_t c = tile < ntiles ? end[tile] - start[tile] : 0u;
+            v[k] = (c + chunk - 1) / chunk;
             s += v[k];
+            placed += c;
         }
         // inclusive scan of s: within each wave by shuffles, then over the 16 wave totals
         uint32_t x = s;
@@ -110,11 +115,18 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ sta
         for (int k = 0; k < 4; ++k) {
             const uint32_t tile = b + uint32_t(t) * 4 + k;
             if (tile <= ntiles) item_offset[tile] = run;
+            if (tile < ntiles) write_item_tiles(item_tiles, run, v[k], tile);
             run += v[k];
         }
         carry += wsum[15];
         __syncthreads();
     }
+    // particles on their grid (sorted first): the incremental sort of the next step reads it
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) placed += __shfl_xor(placed, off);
+    if (lane == 0) atomicAdd(&s_placed, placed);
+    __syncthreads();
+    if (t == 0) meta[0] = s_placed;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -200,7 +212,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
     __shared__ uint32_t s_tile, s_chunk;
     const uint32_t item = blockIdx.x;
     if (item >= item_offset[p.ntiles]) return;
-    if (threadIdx.x == 0) item_tile(item_offset, p.ntiles, item, s_tile, s_chunk);
+    if (threadIdx.x == 0) item_tile(p, item_offset, item, s_tile, s_chunk);
     for (int r = threadIdx.x; r < 4 * REG; r += blockDim.x) acc[r] = 0ull;
     for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
     __syncthreads();
@@ -672,7 +684,8 @@ static hipError_t sort_dispatch(const LaunchArgs& a) {
         if (e != hipSuccess) return e;
         k_tile_bounds<<<blocks_for(p.n, 256), 256, 0, s>>>(a.keys, p.n, a.tile_start, a.tile_end);
     }
-    k_items<<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset);
+    k_items<<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset, p.item_tiles,
+                               a.meta);
     return hipGetLastError();
 }
 
