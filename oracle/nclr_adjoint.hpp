@@ -25,6 +25,7 @@ template <typename T, int D>
 struct StepGrads {
     std::vector<T> x, v, F, C;  // particle-major: x[n][D], F[n][D][D]
     T E = T(0);
+    T E_abs = T(0);  // sum over particles of |their contribution to E|: the scale of E's rounding
     explicit StepGrads(size_t n = 0) : x(n * D, T(0)), v(n * D, T(0)), F(n * D * D, T(0)), C(n * D * D, T(0)) {}
 };
 
@@ -293,7 +294,9 @@ void adjoint_step_jelly(const std::vector<Particle<T, D>>& ps, int res, T dt, T 
             for (int b = 0; b < D; ++b) in.F[(p * D + a) * D + b] += Fbar[a][b] + Jbar * cof[a][b];
         const T lambar = sG * (J - T(1)) * J;
         const T e = T(0.3f);
-        in.E += e * (mubar / (T(2) * (T(1) + nu)) + lambar * nu / ((T(1) + nu) * (T(1) - T(2) * nu)));
+        const T ep = e * (mubar / (T(2) * (T(1) + nu)) + lambar * nu / ((T(1) + nu) * (T(1) - T(2) * nu)));
+        in.E += ep;
+        in.E_abs += ep < T(0) ? -ep : ep;
     }
 }
 

@@ -226,6 +226,7 @@ class Grads:
     F: np.ndarray
     C: np.ndarray
     E: float = 0.0
+    E_abs: float = 0.0  # sum of the particles' |contributions| to E (the scale of its rounding)
 
 
 def adjoint_jelly(state: State, grads_out: Grads, res: int = 64, dt: float = 1e-4, E: float = 1e4, nu: float = 0.2,
@@ -236,15 +237,15 @@ def adjoint_jelly(state: State, grads_out: Grads, res: int = 64, dt: float = 1e-
     n, d = s.n, s.dim
     g = [np.ascontiguousarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")]
     outs = [np.zeros_like(a) for a in g]
-    e = ctypes.c_double(0.0)
+    e = np.zeros(2, np.float64)
     lib = _lib()
     rc = lib.oracle_adjoint_jelly_f64(
         ctypes.c_int(d), ctypes.c_int(res), ctypes.c_double(dt), ctypes.c_double(E), ctypes.c_double(nu),
         ctypes.c_double(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(n), _ptr(s.x), _ptr(s.v), _ptr(s.F),
         _ptr(s.C), _ptr(s.Jp), _ptr(s.mass), _ptr(s.volume), *(_ptr(a) for a in g), *(_ptr(a) for a in outs),
-        ctypes.byref(e))
+        _ptr(e))
     _check(rc, lib)
-    return Grads(*outs, E=e.value)
+    return Grads(*outs, E=float(e[0]), E_abs=float(e[1]))
 
 
 def rollout_vjp_jelly(state: State, grads_out: Grads, steps: int, res: int = 64, dt: float = 1e-4, E: float = 1e4,
@@ -256,11 +257,12 @@ def rollout_vjp_jelly(state: State, grads_out: Grads, steps: int, res: int = 64,
     for _ in range(steps - 1):
         states.append(run(states[-1], JELLY, steps=1, fp64=True, **kw))
     g = Grads(*(np.asarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")))
-    total_E = 0.0
+    total_E = total_abs = 0.0
     for s in reversed(states):
         g = adjoint_jelly(s, g, **kw)
         total_E += g.E
-    g.E = total_E
+        total_abs += g.E_abs
+    g.E, g.E_abs = total_E, total_abs
     return g
 
 

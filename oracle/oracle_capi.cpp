@@ -206,7 +206,8 @@ int adjoint(int res, double dt, double E, double nu, double g, int idm, int64_t 
         std::copy(in.v.begin(), in.v.end(), iv);
         std::copy(in.F.begin(), in.F.end(), iF);
         std::copy(in.C.begin(), in.C.end(), iC);
-        *iE = in.E;
+        iE[0] = in.E;
+        iE[1] = in.E_abs;
     });
 }
 
@@ -279,7 +280,8 @@ const char* oracle_last_error(void) { return g_err.c_str(); }
 DEFINE_FOR(float, f32)
 DEFINE_FOR(double, f64)
 
-// FP64 vector-Jacobian product of one jelly step (nclr_adjoint.hpp).
+// FP64 vector-Jacobian product of one jelly step (nclr_adjoint.hpp); iE[0] = dL/dE, iE[1] = the
+// sum of the particles' |contributions| to it.
 int oracle_adjoint_jelly_f64(int dim, int res, double dt, double E, double nu, double g, int idm, int64_t n,
                              const double* x, const double* v, const double* F, const double* C, const double* Jp,
                              const double* mass, const double* volume, const double* gx, const double* gv,
