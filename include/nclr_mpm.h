@@ -208,6 +208,56 @@ int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float*
  * src/nclr.h:77-78 does). */
 int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu);
 
+/* ---- Scene generators and viewer hooks on the device ------------------------------------------
+ *
+ * The reference's scene helpers (src/nclr_math.h:22-39, 100-147) and pt_3d_to_2d (149-167) as
+ * device kernels.  Generators write a state image (the layout mpm_set_state_device takes:
+ * channel c of particle i at image[c*n + i], channels x[dim], v[dim], F[dim*dim], C[dim*dim]) with
+ * v from the shape, F = diag<dim>(1) (diag(1, 1, 0) in 3D for identity_mode 0, I for 1) and C = 0.
+ * `image`, `pos` and `out` are device memory; the calls run on the device's null stream and return
+ * when the device is done.  A state built this way enters an engine through mpm_create (any x,
+ * e.g. zeros) followed by mpm_set_state_device. */
+
+/* One cube of particles: a res[0] x res[1] (x res[2]) lattice, axis a spanning
+ * LinSpaced(res[a], lo[a], hi[a]), last axis fastest: cube<dim>(res, min, max) of
+ * src/nclr_math.h:100-129 is res[a] = res, lo[a] = min, hi[a] = max.  v: initial velocity. */
+typedef struct mpm_cube {
+    int32_t res[3];
+    float lo[3], hi[3];
+    float v[3];
+} mpm_cube;
+
+/* The lattices of `ncubes` cubes, one after another; n must be the total point count. */
+int mpm_gen_cubes(int dim, int ncubes, const mpm_cube* cubes, int identity_mode, int64_t n, float* image,
+                  int device);
+
+/* The reference's random stream: nc_rand_int (src/nclr_math.h:22-29) is xorshift128 on the state
+ * {x, y, z, w}, seeded {123456789, 362436069, 521288629, 88675123}; nc_rand = u32 * 2^-32 in float.
+ * mpm_rand_advance moves `state` k draws ahead (a GF(2) jump, any k). */
+int mpm_rand_advance(uint32_t state[4], uint64_t k);
+/* out[k] = the k-th next nc_rand() for k < count; `state` advances by count. */
+int mpm_gen_rand(int64_t count, uint32_t state[4], float* out, int device);
+/* sample<dim>(center) (src/nclr_math.h:141-147, which has no return statement; here the points
+ * are the result) for each of the ncenters host centres (ncenters x dim floats): per_center
+ * particles (1000 in the reference) at (randvec<dim>() * 2 - 1) * half_width + center (0.08),
+ * drawing from `state`, which advances by ncenters * per_center * dim.  n = ncenters * per_center. */
+int mpm_gen_sample(int dim, int ncenters, const float* centers, int per_center, float half_width, uint32_t state[4],
+                   int identity_mode, float* image, int device);
+
+/* gyroid(k, t, pos) (src/nclr_math.h:131-139) at n positions (n x 3): out[i], float. */
+int mpm_gyroid(int64_t n, const float* pos, float k, float t, float* out, int device);
+/* The points of the 3D lattice `box` where gyroid(k, t, p) <= 0 (thickness == 0: the solid
+ * network) or |gyroid(k, t, p)| <= thickness (a sheet), in lattice order.  *n_out receives the
+ * count; image may be NULL to count only, else it must hold exactly *n_out particles: pass that
+ * count in `capacity` (a different value is an error). */
+int mpm_gen_gyroid(const mpm_cube* box, float k, float t, float thickness, int identity_mode, float* image,
+                   int64_t capacity, int64_t* n_out, int device);
+
+/* pt_3d_to_2d (src/nclr_math.h:154-167) of n positions (n x 3) into out (n x 2).  centered = 0
+ * computes what the reference computes (its centred copy of the point is unused); 1 rotates
+ * the point about (0.5, 0.5, 0.5). */
+int mpm_project_points(int64_t n, const float* pos, int centered, float* out, int device);
+
 /* Releases the device memory, graphs and stream of h. */
 int mpm_destroy(mpm_handle h);
 

@@ -50,6 +50,15 @@ class mpm_scene(ctypes.Structure):
     ]
 
 
+class mpm_cube(ctypes.Structure):
+    _fields_ = [
+        ("res", ctypes.c_int32 * 3),
+        ("lo", ctypes.c_float * 3),
+        ("hi", ctypes.c_float * 3),
+        ("v", ctypes.c_float * 3),
+    ]
+
+
 class EngineError(RuntimeError):
     """Error reported by the HIP engine."""
 
@@ -141,6 +150,21 @@ def lib() -> ctypes.CDLL:
     L.mpm_backward_segment.restype = i32
     L.mpm_set_scene_elasticity.argtypes = [H, i32, f32, f32]
     L.mpm_set_scene_elasticity.restype = i32
+    u32x4 = ctypes.POINTER(ctypes.c_uint32)
+    L.mpm_gen_cubes.argtypes = [i32, i32, ctypes.POINTER(mpm_cube), i32, i64, _vp, i32]
+    L.mpm_gen_cubes.restype = i32
+    L.mpm_rand_advance.argtypes = [u32x4, ctypes.c_uint64]
+    L.mpm_rand_advance.restype = i32
+    L.mpm_gen_rand.argtypes = [i64, u32x4, _vp, i32]
+    L.mpm_gen_rand.restype = i32
+    L.mpm_gen_sample.argtypes = [i32, i32, _vp, i32, f32, u32x4, i32, _vp, i32]
+    L.mpm_gen_sample.restype = i32
+    L.mpm_gyroid.argtypes = [i64, _vp, f32, f32, _vp, i32]
+    L.mpm_gyroid.restype = i32
+    L.mpm_gen_gyroid.argtypes = [ctypes.POINTER(mpm_cube), f32, f32, f32, i32, _vp, i64, ctypes.POINTER(i64), i32]
+    L.mpm_gen_gyroid.restype = i32
+    L.mpm_project_points.argtypes = [i64, _vp, i32, _vp, i32]
+    L.mpm_project_points.restype = i32
     L.mpm_destroy.argtypes = [H]
     L.mpm_destroy.restype = i32
     L.mpm_last_error.argtypes = []

@@ -1032,6 +1032,145 @@ int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu) {
     return h->set_elasticity(scene, E, nu);
 }
 
+}  // extern "C"
+
+// ---- scene generators and viewer hooks: null stream, synchronous --------------------------------
+
+namespace {
+
+// Uploads `data` (count elements) to a temporary device buffer.
+template <typename T>
+hipError_t upload_tmp(const T* data, size_t count, T** dev) {
+    hipError_t e = dmalloc(dev, count);
+    if (e == hipSuccess && count) e = hipMemcpy(*dev, data, count * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+int finish(hipError_t e, std::initializer_list<void*> bufs, const char* what) {
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+bool valid_cube(const mpm_cube& c, int dim) {
+    for (int a = 0; a < dim; ++a)
+        if (c.res[a] < 1 || !std::isfinite(c.lo[a]) || !std::isfinite(c.hi[a]) || !std::isfinite(c.v[a])) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_gen_cubes(int dim, int ncubes, const mpm_cube* cubes, int identity_mode, int64_t n, float* image,
+                  int device) {
+    if (dim != 2 && dim != 3) return fail(MPM_ERR_INVALID, "dim must be 2 or 3");
+    if (ncubes < 0 || (ncubes > 0 && !cubes)) return fail(MPM_ERR_INVALID, "bad cube list");
+    if (identity_mode != 0 && identity_mode != 1) return fail(MPM_ERR_INVALID, "identity_mode must be 0 or 1");
+    std::vector<int64_t> offsets(size_t(ncubes) + 1, 0);
+    for (int c = 0; c < ncubes; ++c) {
+        if (!valid_cube(cubes[c], dim)) return fail(MPM_ERR_INVALID, "cube " + std::to_string(c) + " is invalid");
+        int64_t pts = 1;
+        for (int a = 0; a < dim; ++a) pts *= cubes[c].res[a];
+        offsets[size_t(c) + 1] = offsets[size_t(c)] + pts;
+    }
+    if (n != offsets.back()) return fail(MPM_ERR_INVALID, "n must equal the total lattice point count");
+    if (n == 0) return MPM_OK;
+    if (!image) return fail(MPM_ERR_INVALID, "NULL image");
+    HIP_TRY(hipSetDevice(device));
+    mpm_cube* dc = nullptr;
+    int64_t* doff = nullptr;
+    hipError_t e = upload_tmp(cubes, size_t(ncubes), &dc);
+    if (e == hipSuccess) e = upload_tmp(offsets.data(), offsets.size(), &doff);
+    if (e == hipSuccess) e = nclr_host::launch_gen_cubes(dim, dc, doff, ncubes, identity_mode, n, image, nullptr);
+    return finish(e, {dc, doff}, "mpm_gen_cubes");
+}
+
+int mpm_rand_advance(uint32_t state[4], uint64_t k) {
+    if (!state) return fail(MPM_ERR_INVALID, "NULL state");
+    nclr_host::rand_jump(state, k);
+    return MPM_OK;
+}
+
+int mpm_gen_rand(int64_t count, uint32_t state[4], float* out, int device) {
+    if (!state || count < 0 || (count > 0 && !out)) return fail(MPM_ERR_INVALID, "bad arguments");
+    if (count == 0) return MPM_OK;
+    HIP_TRY(hipSetDevice(device));
+    const std::vector<uint4> starts = nclr_host::rand_starts(state, count);
+    uint4* ds = nullptr;
+    hipError_t e = upload_tmp(starts.data(), starts.size(), &ds);
+    if (e == hipSuccess) e = nclr_host::launch_gen_rand(ds, count, out, nullptr);
+    const int rc = finish(e, {ds}, "mpm_gen_rand");
+    if (rc == MPM_OK) nclr_host::rand_jump(state, uint64_t(count));
+    return rc;
+}
+
+int mpm_gen_sample(int dim, int ncenters, const float* centers, int per_center, float half_width, uint32_t state[4],
+                   int identity_mode, float* image, int device) {
+    if (dim != 2 && dim != 3) return fail(MPM_ERR_INVALID, "dim must be 2 or 3");
+    if (ncenters < 0 || per_center < 0 || !state || (ncenters > 0 && !centers))
+        return fail(MPM_ERR_INVALID, "bad arguments");
+    if (identity_mode != 0 && identity_mode != 1) return fail(MPM_ERR_INVALID, "identity_mode must be 0 or 1");
+    const int64_t n = int64_t(ncenters) * per_center;
+    if (n == 0) return MPM_OK;
+    if (!image) return fail(MPM_ERR_INVALID, "NULL image");
+    HIP_TRY(hipSetDevice(device));
+    const std::vector<uint4> starts = nclr_host::rand_starts(state, n * dim);
+    uint4* ds = nullptr;
+    float* dc = nullptr;
+    hipError_t e = upload_tmp(starts.data(), starts.size(), &ds);
+    if (e == hipSuccess) e = upload_tmp(centers, size_t(ncenters) * dim, &dc);
+    if (e == hipSuccess)
+        e = nclr_host::launch_gen_sample(dim, ds, dc, per_center, half_width, identity_mode, n, image, nullptr);
+    const int rc = finish(e, {ds, dc}, "mpm_gen_sample");
+    if (rc == MPM_OK) nclr_host::rand_jump(state, uint64_t(n * dim));
+    return rc;
+}
+
+int mpm_gyroid(int64_t n, const float* pos, float k, float t, float* out, int device) {
+    if (n < 0 || (n > 0 && (!pos || !out))) return fail(MPM_ERR_INVALID, "bad arguments");
+    if (n == 0) return MPM_OK;
+    HIP_TRY(hipSetDevice(device));
+    return finish(nclr_host::launch_eval_gyroid(n, pos, k, t, out, nullptr), {}, "mpm_gyroid");
+}
+
+int mpm_gen_gyroid(const mpm_cube* box, float k, float t, float thickness, int identity_mode, float* image,
+                   int64_t capacity, int64_t* n_out, int device) {
+    if (!box || !n_out || !valid_cube(*box, 3)) return fail(MPM_ERR_INVALID, "bad box");
+    if (identity_mode != 0 && identity_mode != 1) return fail(MPM_ERR_INVALID, "identity_mode must be 0 or 1");
+    if (!(thickness >= 0.0f) || !std::isfinite(k) || !std::isfinite(t)) return fail(MPM_ERR_INVALID, "bad k, t or thickness");
+    if (int64_t(box->res[0]) * box->res[1] * box->res[2] > (int64_t(1) << 40))
+        return fail(MPM_ERR_INVALID, "box has too many lattice points");
+    HIP_TRY(hipSetDevice(device));
+    const int64_t nb = nclr_host::gyroid_blocks(*box);
+    uint32_t* counts = nullptr;
+    int64_t* offsets = nullptr;
+    hipError_t e = dmalloc(&counts, size_t(nb));
+    if (e == hipSuccess) e = dmalloc(&offsets, size_t(nb) + 1);
+    if (e == hipSuccess) e = nclr_host::launch_gyroid_count(*box, k, t, thickness, counts, offsets, nullptr);
+    int64_t total = 0;
+    if (e == hipSuccess) e = hipMemcpy(&total, offsets + nb, sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+        *n_out = total;
+        if (image && capacity != total) {
+            (void)finish(e, {counts, offsets}, "mpm_gen_gyroid");
+            return fail(MPM_ERR_INVALID, "capacity must equal the point count (" + std::to_string(total) + ")");
+        }
+        if (image)
+            e = nclr_host::launch_gyroid_scatter(*box, k, t, thickness, offsets, identity_mode, total, image, nullptr);
+    }
+    return finish(e, {counts, offsets}, "mpm_gen_gyroid");
+}
+
+int mpm_project_points(int64_t n, const float* pos, int centered, float* out, int device) {
+    if (n < 0 || (n > 0 && (!pos || !out))) return fail(MPM_ERR_INVALID, "bad arguments");
+    if (n == 0) return MPM_OK;
+    HIP_TRY(hipSetDevice(device));
+    return finish(nclr_host::launch_project(n, pos, centered ? 1 : 0, out, nullptr), {}, "mpm_project_points");
+}
+
 int mpm_destroy(mpm_handle h) {
     if (!h) return MPM_OK;
     (void)hipSetDevice(h->opts.device);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/nclr_mpm.h"
 
@@ -140,6 +141,21 @@ hipError_t launch_active_sum(const LaunchArgs& a, uint32_t first_tile, uint32_t 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* hist,
                             int64_t n, int bits, bool identity_values, hipStream_t s);
 int64_t sort_tiles(int64_t n);
+// scene generators and the viewer projection (mpm_scene.hip)
+void rand_jump(uint32_t state[4], uint64_t k);
+std::vector<uint4> rand_starts(const uint32_t state[4], int64_t count);
+hipError_t launch_gen_cubes(int dim, const mpm_cube* cubes, const int64_t* offsets, int ncubes, int identity_mode,
+                            int64_t n, float* img, hipStream_t s);
+hipError_t launch_gen_rand(const uint4* starts, int64_t count, float* out, hipStream_t s);
+hipError_t launch_gen_sample(int dim, const uint4* starts, const float* centers, int per_center, float half,
+                             int identity_mode, int64_t n, float* img, hipStream_t s);
+hipError_t launch_eval_gyroid(int64_t n, const float* pos, float k, float t, float* out, hipStream_t s);
+int64_t gyroid_blocks(const mpm_cube& box);
+hipError_t launch_gyroid_count(const mpm_cube& box, float k, float t, float thickness, uint32_t* counts,
+                               int64_t* offsets, hipStream_t s);
+hipError_t launch_gyroid_scatter(const mpm_cube& box, float k, float t, float thickness, const int64_t* offsets,
+                                 int identity_mode, int64_t n, float* img, hipStream_t s);
+hipError_t launch_project(int64_t n, const float* pos, int centered, float* out, hipStream_t s);
 int64_t max_items(int64_t n, uint32_t ntiles);
 int num_fields(int dim);
 int region_nodes(int dim);

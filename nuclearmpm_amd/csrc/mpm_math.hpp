@@ -82,10 +82,11 @@ NCLR_DI Rot make_jacobi(float x, float y, float z) {
     if (deno < kFltMin) return {1.0f, 0.0f};
     const float tau = (x - z) / deno;
     const float w = sqrtf(tau * tau + 1.0f);
-    const float t = tau > 0.0f ? 1.0f / (tau + w) : 1.0f / (tau - w);
+    const float t = 1.0f / (tau > 0.0f ? tau + w : tau - w);
     const float sign_t = t > 0.0f ? 1.0f : -1.0f;
     const float n = 1.0f / sqrtf(t * t + 1.0f);
-    const float s = -sign_t * (y / fabsf(y)) * fabsf(t) * n;
+    // y / |y| is exactly +-1 for the finite, nonzero y that reaches here
+    const float s = -sign_t * copysignf(1.0f, y) * fabsf(t) * n;
     return {n, s};
 }
 

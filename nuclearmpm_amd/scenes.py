@@ -94,6 +94,17 @@ def cube(dim: int, res: int, lo: float, hi: float) -> np.ndarray:
 
 
 @dataclass
+class CubeSpec:
+    """One lattice cube of a scene: res[a] points on axis a over [lo[a], hi[a]] (converted to float32
+    as lattice() does), every particle with velocity v.  The device generator
+    (nuclearmpm_amd.generators) rebuilds a scene's particles from these bit for bit."""
+    res: List[int]
+    lo: List[float]
+    hi: List[float]
+    v: Optional[List[float]] = None
+
+
+@dataclass
 class Scene:
     """One simulation's inputs."""
     particles: Particles
@@ -135,16 +146,17 @@ def four_cubes_3d(seed: int = 0, side_res: int = 64, cubes: int = 4, side: float
     one velocity uniform in [-1,1]^3 per cube; res 128, dt 1e-4, 1000 steps."""
     rng = XorShift128(seed)
     centres = _cube_centres(rng, cubes, 3, 0.2, 0.8, side)
-    xs, vs = [], []
+    xs, vs, cubes = [], [], []
     for c in centres:
         v = np.array([rng.uniform(-1.0, 1.0) for _ in range(3)], np.float32)
         x = lattice([side_res] * 3, list(c - side / 2), list(c + side / 2))
         xs.append(x)
         vs.append(np.broadcast_to(v, x.shape))
+        cubes.append(CubeSpec([side_res] * 3, list(c - side / 2), list(c + side / 2), list(v)))
     x = np.concatenate(xs)
     p = Particles.from_positions(x, v=np.concatenate(vs))
     return Scene(p, MaterialModel.kSnow, res=128, steps=1000, name="four_cubes_3d",
-                 extra={"centres": np.array(centres)})
+                 extra={"centres": np.array(centres), "cubes": cubes})
 
 
 def batched_2d(seed: int = 1, scenes: int = 256, side: float = 0.2) -> List[Scene]:
@@ -155,14 +167,15 @@ def batched_2d(seed: int = 1, scenes: int = 256, side: float = 0.2) -> List[Scen
     mats = [MaterialModel.kLiquid, MaterialModel.kSnow, MaterialModel.kJelly]
     for s in range(scenes):
         ncubes = rng.randint(1, 4)
-        xs = []
+        xs, cubes = [], []
         for _ in range(ncubes):
             r = rng.randint(20, 80)
             corner = [rng.uniform(0.1, 0.9 - side) for _ in range(2)]
             xs.append(lattice([r, r], corner, [c + side for c in corner]))
+            cubes.append(CubeSpec([r, r], corner, [c + side for c in corner]))
         mat = mats[min(2, int(rng.uniform() * 3))]
         out.append(Scene(Particles.from_positions(np.concatenate(xs)), mat, res=128, steps=2000,
-                         name=f"batched_2d[{s}]"))
+                         name=f"batched_2d[{s}]", extra={"cubes": cubes}))
     return out
 
 
@@ -171,7 +184,8 @@ def jelly_2d_diff() -> Scene:
     loss = |centre of mass - (0.7, 0.3)|^2."""
     x = cube(2, 140, 0.3, 0.5)
     return Scene(Particles.from_positions(x), MaterialModel.kJelly, res=128, E=5e3, steps=512,
-                 name="jelly_2d_diff", extra={"target": np.array([0.7, 0.3], np.float32), "checkpoint_every": 64})
+                 name="jelly_2d_diff", extra={"target": np.array([0.7, 0.3], np.float32), "checkpoint_every": 64,
+                                              "cubes": [CubeSpec([140, 140], [0.3, 0.3], [0.5, 0.5])]})
 
 
 def jelly_3d_batched(seed: int = 2, scenes: int = 8, side_res: int = 64, side: float = 0.25) -> List[Scene]:
@@ -185,5 +199,7 @@ def jelly_3d_batched(seed: int = 2, scenes: int = 8, side_res: int = 64, side: f
         target = np.array([rng.uniform(0.2, 0.8) for _ in range(3)], np.float32)
         x = lattice([side_res] * 3, list(corner), list(corner + side))
         out.append(Scene(Particles.from_positions(x), MaterialModel.kJelly, res=128, E=E, steps=256,
-                         name=f"jelly_3d_batched[{s}]", extra={"target": target, "checkpoint_every": 32}))
+                         name=f"jelly_3d_batched[{s}]",
+                         extra={"target": target, "checkpoint_every": 32,
+                                "cubes": [CubeSpec([side_res] * 3, list(corner), list(corner + side))]}))
     return out

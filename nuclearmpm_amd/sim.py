@@ -126,6 +126,19 @@ class Particles:
         return Particles(*(getattr(self, k).copy() for k in ("x", "v", "F", "C", "Jp", "mass", "volume", "c")))
 
 
+def _image_dim(image) -> int:
+    rows = int(image.shape[0])
+    for d in (2, 3):
+        if rows == 2 * d + 2 * d * d:
+            return d
+    raise ValueError(f"a state image has 12 (2D) or 24 (3D) rows, not {rows}")
+
+
+def _placeholder(n: int, dim: int) -> Particles:
+    # positions are overwritten from the image before any step
+    return Particles.from_positions(np.zeros((n, dim), np.float32))
+
+
 def default_options(**overrides) -> mpm_options:
     o = mpm_options()
     lib().mpm_default_options(ctypes.byref(o))
@@ -150,6 +163,33 @@ class _Engine:
                                        graph=int(graph), autotune=int(autotune), timing=int(timing), device=device,
                                        fast_svd=int(bool(fast_svd)))
         self._cache = None
+
+    def set_state_image(self, image) -> None:
+        """Sets x, v, F, C of every particle from a device state image: a float32 tensor of shape
+        (2d + 2d^2, n) on the engine's device, rows x, v, F, C (row-major), particle order (the
+        format of nuclearmpm_amd.generators).  Jp, mass and volume are kept."""
+        import torch
+
+        rows = 2 * self._dim + 2 * self._dim * self._dim
+        if (not isinstance(image, torch.Tensor) or image.dtype != torch.float32 or image.device.type != "cuda"
+                or tuple(image.shape) != (rows, self._n)):
+            raise ValueError(f"state image must be a float32 cuda tensor of shape ({rows}, {self._n})")
+        image = image.contiguous()
+        torch.cuda.synchronize(image.device)
+        self._cache = None
+        check(lib().mpm_set_state_device(self._handle(), image.data_ptr() if self._n else None))
+        check(lib().mpm_synchronize(self._handle()))
+
+    def state_image(self):
+        """The current x, v, F, C as a device state image (particle order)."""
+        import torch
+
+        rows = 2 * self._dim + 2 * self._dim * self._dim
+        out = torch.empty((rows, self._n), dtype=torch.float32, device=f"cuda:{self.options.device}")
+        check(lib().mpm_synchronize(self._handle()))
+        check(lib().mpm_get_state_device(self._handle(), out.data_ptr() if self._n else None))
+        check(lib().mpm_synchronize(self._handle()))
+        return out
 
     def advance(self, steps: int = 1) -> None:
         """advance() (src/nclr.h:80-84), `steps` times for every scene; queued without waiting."""
@@ -301,6 +341,19 @@ class MPMSimulation(_Engine):
         self.dx = np.float32(dx.value)
         self.inv_dx = np.float32(inv_dx.value)
 
+    @classmethod
+    def from_image(cls, image, model: MaterialModel, res: int = 64, dt: float = 1e-4, E: float = 1e4,
+                   nu: float = 0.2, gravity: float = -100.0, *, autotune: bool = False, **kw) -> "MPMSimulation":
+        """A simulation whose particles come from a device state image (nuclearmpm_amd.generators);
+        Jp, mass and volume take the Particle ctor defaults (1)."""
+        dim = _image_dim(image)
+        n = int(image.shape[1])
+        sim = cls(_placeholder(n, dim), model, res, dt, E, nu, gravity, **kw)
+        sim.set_state_image(image)
+        if autotune:
+            sim.autotune()
+        return sim
+
     def particles(self) -> Particles:
         """particles() (src/nclr.h:86): the current state, original particle order."""
         return self._all_particles()
@@ -357,6 +410,26 @@ class MPMBatch(_Engine):
                                      ptr(allp.C), ptr(allp.Jp), ptr(allp.mass), ptr(allp.volume),
                                      ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_image(cls, image, scenes: Sequence, *, autotune: bool = False, **kw) -> "MPMBatch":
+        """A batch whose particles come from a device state image (scenes concatenated in order).
+        Each scene gives its particle count (`n`, or len(particles)) and the ctor arguments
+        material, res, dt, E, nu, gravity (defaults as in SceneSpec)."""
+        dim = _image_dim(image)
+        specs = []
+        for s in scenes:
+            n = int(s.n) if hasattr(s, "n") and not hasattr(s, "particles") else int(s.particles.n)
+            specs.append(SceneSpec(_placeholder(n, dim), MaterialModel(s.material), getattr(s, "res", 64),
+                                   getattr(s, "dt", 1e-4), getattr(s, "E", 1e4), getattr(s, "nu", 0.2),
+                                   getattr(s, "gravity", -100.0)))
+        if sum(sp.particles.n for sp in specs) != int(image.shape[1]):
+            raise ValueError("the scenes' particle counts do not add up to the image's columns")
+        batch = cls(specs, **kw)
+        batch.set_state_image(image)
+        if autotune:
+            batch.autotune()
+        return batch
 
     @property
     def num_scenes(self) -> int:
