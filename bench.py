@@ -168,6 +168,44 @@ def fwd_bwd(which: str):
     }
 
 
+def scene_generation():
+    """The 3D snow workload's 1,048,576-particle state built on the device (lattices, velocities,
+    F, C: a 24 x n float image, 100.7 MB written) against the host scene generator plus the
+    upload of the same image."""
+    import numpy as np
+    import torch
+
+    from nuclearmpm_amd import generators, scenes
+
+    sc = scenes.four_cubes_3d()
+    generators.scene_image(sc)  # warm the kernels up
+    dev_ms, kern_ms = [], []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        img = generators.scene_image(sc)
+        e1.record()
+        torch.cuda.synchronize()
+        dev_ms.append(1e3 * (time.perf_counter() - t0))
+        kern_ms.append(e0.elapsed_time(e1))
+    host_ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        h = scenes.four_cubes_3d().particles
+        n = h.x.shape[0]
+        host = np.concatenate([h.x.T, h.v.T, h.F.reshape(n, -1).T, h.C.reshape(n, -1).T])
+        up = torch.as_tensor(host, device="cuda")
+        torch.cuda.synchronize()
+        host_ms.append(1e3 * (time.perf_counter() - t0))
+    same = bool(torch.equal(img, up))
+    nbytes = img.numel() * 4
+    return {"particles": int(img.shape[1]), "image_bytes": nbytes, "device_ms": float(np.median(dev_ms)),
+            "device_stream_ms": float(np.median(kern_ms)),
+            "device_GBs": nbytes / (float(np.median(kern_ms)) * 1e-3) / 1e9,
+            "host_plus_upload_ms": float(np.median(host_ms)), "bitwise_equal": same}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,7 +320,8 @@ def main() -> int:
     if not args.no_extra:
         legs = (("batched", lambda: batched_vs_sequential(200)),
                 ("fwd_bwd_2d", lambda: fwd_bwd("jelly_2d_diff")),
-                ("fwd_bwd_3d", lambda: fwd_bwd("jelly_3d_batched")))
+                ("fwd_bwd_3d", lambda: fwd_bwd("jelly_3d_batched")),
+                ("scene_gen", scene_generation))
         for name, fn in legs:
             try:
                 out[name] = fn()

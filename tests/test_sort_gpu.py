@@ -43,3 +43,59 @@ def test_sort_ignores_bits_above_request_and_handles_large_n():
     perm = sort_keys(keys, 13)
     assert np.array_equal(perm, expect(keys, 13))
     assert sort_keys(np.zeros(0, np.uint32), 8).shape == (0,)
+
+
+def slot_order(sim):
+    """Particle index of every storage slot: the engine's particle -> slot permutation of state
+    images (mpm_grad_from_particle_order) applied to the particle indices."""
+    import torch
+
+    from nuclearmpm_amd._lib import check, lib
+
+    d, n = sim.dim, sim.n
+    src = torch.arange(n, dtype=torch.float32, device="cuda").repeat(2 * d + 2 * d * d, 1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    check(lib().mpm_grad_from_particle_order(sim._handle(), src.data_ptr(), dst.data_ptr()))
+    check(lib().mpm_synchronize(sim._handle()))
+    return dst[0].cpu().numpy().astype(np.int64)
+
+
+def tile_keys(x, res, dim):
+    """Tile of every particle's stencil base, x-major (8^3 cells per tile in 3D, 16^2 in 2D)."""
+    log = 4 if dim == 2 else 3
+    nt = ((res - 2) >> log) + 1
+    s = (x.astype(np.float32) * np.float32(res)).astype(np.float32)
+    base = (s - np.float32(0.5)).astype(np.float32).astype(np.int64)
+    key = np.zeros(len(x), np.int64)
+    for a in range(dim):
+        key = key * nt + (base[:, a] >> log)
+    return key
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_incremental_sort_is_the_stable_sort_by_tile(dim):
+    """Every step's storage order must be the stable sort, by the tile of the state before the
+    step, of the previous step's order (the first step sorts the particle order)."""
+    from nuclearmpm_amd import MaterialModel, MPMSimulation
+    from nuclearmpm_amd.sim import Particles
+
+    rng = np.random.default_rng(10 + dim)
+    res, n = (64, 6000) if dim == 2 else (48, 6000)
+    x = rng.uniform(0.15, 0.85, (n, dim)).astype(np.float32)
+    # up to 0.6 cells per step on each axis: many tile crossings every step
+    v = (rng.uniform(-1.0, 1.0, (n, dim)) * (0.6 / (res * 1e-4))).astype(np.float32)
+    with MPMSimulation(Particles.from_positions(x, v=v), MaterialModel.kJelly, res=res) as sim:
+        prev = np.arange(n)
+        moved = 0
+        for step in range(20):
+            key = tile_keys(sim.particles().x, res, dim)
+            sim.advance(1)
+            order = slot_order(sim)
+            want = prev[np.argsort(key[prev], kind="stable")]
+            assert np.array_equal(order, want), f"step {step}"
+            if step:
+                moved += int(np.count_nonzero(key != last_key))
+            prev, last_key = order, key
+        print(f"[sort] {dim}D: {moved} tile changes over 19 steps", flush=True)
+        assert moved > 100
