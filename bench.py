@@ -216,6 +216,9 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and forward+backward legs")
     ap.add_argument("--no-graph", action="store_true", help="launch the step kernels one by one (no hipGraph)")
     ap.add_argument("--fast-svd", action="store_true", help="engine option fast_svd (not bitwise the reference)")
+    ap.add_argument("--tuning", default="auto",
+                    help="P2G particles per work item and G2P workgroup size as 'CHUNKxTHREADS' (e.g. 512x256), "
+                         "or 'auto' to let the engine's autotuner time every shape on the warmed-up state")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM traffic counters)")
     args = ap.parse_args()
@@ -244,6 +247,14 @@ def main() -> int:
     sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity,
                         graph=not args.no_graph, fast_svd=args.fast_svd)
     sim.advance(args.warmup)
+    sim.synchronize()
+    if args.tuning == "auto":
+        tuning = sim.autotune(5)  # untimed; restores the state and the step count
+    else:
+        chunk, threads = (int(v) for v in args.tuning.lower().split("x"))
+        sim.set_tuning(chunk, threads)
+        tuning = sim.tuning()
+    sim.advance(2)
     sim.synchronize()
     t0 = time.perf_counter()
     sim.advance(args.steps)
@@ -302,6 +313,8 @@ def main() -> int:
             "material": "snow",
             "mode": "fwd",
             "hipgraph": not args.no_graph,
+            "tuning": {"p2g_chunk": int(tuning["p2g_chunk"]), "g2p_threads": int(tuning["g2p_threads"]),
+                       "chosen_by": "autotuner" if args.tuning == "auto" else "flag"},
             "svd": "fast (not bitwise the reference)" if args.fast_svd else "nclr_svd order (bitwise the reference's FP32)",
         },
         "roofline": {

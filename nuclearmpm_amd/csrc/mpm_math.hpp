@@ -77,14 +77,26 @@ struct Rot {
     float c, s;
 };
 
+// Correctly rounded sqrt(x) for x >= 1 (or +inf): the hardware square root (within 1 ulp) moved to
+// a neighbour when the residual x - s' s at that neighbour s' says so.  This is the rounding step
+// hipcc emits for an IEEE sqrtf; its small-argument scaling and its special-value fixup do
+// nothing for these arguments and are left out.
+NCLR_DI float sqrt_ge1(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __int_as_float(__float_as_int(s) - 1);
+    const float up = __int_as_float(__float_as_int(s) + 1);
+    const float r = fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    return fmaf(-up, s, x) > 0.0f ? up : r;
+}
+
 NCLR_DI Rot make_jacobi(float x, float y, float z) {
     const float deno = 2.0f * fabsf(y);
     if (deno < kFltMin) return {1.0f, 0.0f};
     const float tau = (x - z) / deno;
-    const float w = sqrtf(tau * tau + 1.0f);
+    const float w = sqrt_ge1(tau * tau + 1.0f);
     const float t = 1.0f / (tau > 0.0f ? tau + w : tau - w);
     const float sign_t = t > 0.0f ? 1.0f : -1.0f;
-    const float n = 1.0f / sqrtf(t * t + 1.0f);
+    const float n = 1.0f / sqrt_ge1(t * t + 1.0f);
     // y / |y| is exactly +-1 for the finite, nonzero y that reaches here
     const float s = -sign_t * copysignf(1.0f, y) * fabsf(t) * n;
     return {n, s};
@@ -103,7 +115,7 @@ NCLR_DI void jacobi_pair(M<D>& W, M<D>& U, M<D>& V, float& max_diag, bool& finis
         rot1 = {1.0f, 0.0f};
     } else {
         const float u = t / d;
-        const float tmp = sqrtf(1.0f + u * u);
+        const float tmp = sqrt_ge1(1.0f + u * u);
         rot1 = {u / tmp, 1.0f / tmp};
     }
     const float a00 = rot1.c * m00 + rot1.s * m10;
