@@ -59,6 +59,16 @@ def cpu_baseline(sc, steps: int):
     p = sc.particles
     st = O.State(x=p.x, v=p.v, F=p.F, C=p.C, Jp=p.Jp, mass=p.mass, volume=p.volume)
     secs = O.time_steps(st, int(sc.material), sc.res, sc.dt, sc.E, sc.nu, sc.gravity, warmup=1, steps=steps)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    msteps = 2 * steps
+    psecs = O.time_steps(st, int(sc.material), sc.res, sc.dt, sc.E, sc.nu, sc.gravity, warmup=1, steps=msteps,
+                         threads=threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": p.n * steps / secs,
         "unit": "particle-steps/s",
@@ -66,6 +76,15 @@ def cpu_baseline(sc, steps: int):
         "kind": "port",
         "sample": f"same workload ({p.n} particles, res {sc.res}): 1 untimed + {steps} timed steps of the "
                   f"restated serial reference step (oracle/, FP32, -O3 -ffast-math), {secs:.1f} s",
+        "cpu_model": model,
+        "multicore": {
+            "value": p.n * msteps / psecs,
+            "unit": "particle-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"1 untimed + {msteps} timed steps of the same step on {threads} OpenMP threads (per-thread "
+                      f"P2G grids summed per node), {psecs:.1f} s",
+        },
     }
 
 

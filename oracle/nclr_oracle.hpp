@@ -36,10 +36,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <exception>
 #include <limits>
 #include <stdexcept>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace oracle {
 
@@ -365,7 +369,7 @@ class Simulation {
     int identity_mode() const { return identity_mode_; }
     T allowed_velocity() const { return T(double(dx_) * 0.9 / double(dt_)); }
 
-    // Flat index of node base+off, checked like cell_at (for callers outside the class).
+    // Flat index of node base+off, checked like cell_in (for callers outside the class).
     int64_t node_index(const int base[D], const int off[D]) const {
         int64_t idx = 0;
         for (int a = 0; a < D; ++a) idx = idx * (res_ + 1) + int64_t(base[a] + off[a]);
@@ -386,67 +390,120 @@ class Simulation {
 
     // nclr.h:104-165.  Grid reallocated (zeroed) each call.
     void p2g() {
-        cells_.assign(num_nodes(), Cell<T, D>{});
-        for (auto& c : cells_) {
-            for (int a = 0; a < D; ++a) c.v[a] = T(0);
-            c.m = T(0);
-        }
-        for (size_t pp = 0; pp < particles_.size(); ++pp) {
-            const auto& p = particles_[pp];
-            int base[D];
-            T fx[D], w[D][3];
-            base_and_weights(p.x, base, fx, w);
-            T affine[D][D];
-            first_piola_kirchoff_stress(p, affine);
-            T mv[D];
-            for (int a = 0; a < D; ++a) mv[a] = p.v[a] * p.mass;
-            for_each_offset([&](const int off[D]) {
-                T dpos[D];
-                for (int a = 0; a < D; ++a) dpos[a] = (T(off[a]) - fx[a]) * dx_;
-                T weight = w[0][off[0]];
-                for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
-                Cell<T, D>& cell = cell_at(base, off);
-                for (int r = 0; r < D; ++r) {
-                    T ad = affine[r][0] * dpos[0];
-                    for (int c = 1; c < D; ++c) ad = ad + affine[r][c] * dpos[c];
-                    cell.v[r] = cell.v[r] + weight * (mv[r] + ad);
-                }
-                cell.m = cell.m + weight * p.mass;
-            });
-        }
+        zero_grid(cells_);
+        for (size_t pp = 0; pp < particles_.size(); ++pp) deposit(particles_[pp], cells_);
+    }
+
+    // One particle's 3^D contributions (nclr.h:110-158) added into `grid`.
+    void deposit(const Particle<T, D>& p, std::vector<Cell<T, D>>& grid) const {
+        int base[D];
+        T fx[D], w[D][3];
+        base_and_weights(p.x, base, fx, w);
+        T affine[D][D];
+        first_piola_kirchoff_stress(p, affine);
+        T mv[D];
+        for (int a = 0; a < D; ++a) mv[a] = p.v[a] * p.mass;
+        for_each_offset([&](const int off[D]) {
+            T dpos[D];
+            for (int a = 0; a < D; ++a) dpos[a] = (T(off[a]) - fx[a]) * dx_;
+            T weight = w[0][off[0]];
+            for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
+            Cell<T, D>& cell = cell_in(grid, base, off);
+            for (int r = 0; r < D; ++r) {
+                T ad = affine[r][0] * dpos[0];
+                for (int c = 1; c < D; ++c) ad = ad + affine[r][c] * dpos[c];
+                cell.v[r] = cell.v[r] + weight * (mv[r] + ad);
+            }
+            cell.m = cell.m + weight * p.mass;
+        });
     }
 
     // nclr.h:263-310
     void grid_op() {
-        const T allowed = T(double(dx_) * 0.9 / double(dt_));
-        int64_t idx = 0;
-        int n[D];
         const int64_t total = num_nodes();
-        for (idx = 0; idx < total; ++idx) {
-            int64_t rem = idx;
-            for (int a = D - 1; a >= 0; --a) {
-                n[a] = int(rem % (res_ + 1));
-                rem /= (res_ + 1);
-            }
-            Cell<T, D>& g = cells_[idx];
-            if (g.m > T(0)) {
-                for (int a = 0; a < D; ++a) g.v[a] = g.v[a] / g.m;
-                g.v[1] = g.v[1] + dt_ * gravity_;
-                for (int a = 0; a < D; ++a) g.v[a] = std::clamp(g.v[a], -allowed, allowed);
-            }
-            for (int a = 0; a < D; ++a) {
-                if ((n[a] < kBoundary && g.v[a] < T(0)) || (n[a] >= (res_ + 1) - kBoundary && g.v[a] > T(0))) {
-                    for (int b = 0; b < D; ++b) g.v[b] = T(0);
-                    g.m = T(0);
-                }
+        for (int64_t idx = 0; idx < total; ++idx) update_node(idx);
+    }
+
+    void update_node(int64_t idx) {
+        const T allowed = T(double(dx_) * 0.9 / double(dt_));
+        int n[D];
+        int64_t rem = idx;
+        for (int a = D - 1; a >= 0; --a) {
+            n[a] = int(rem % (res_ + 1));
+            rem /= (res_ + 1);
+        }
+        Cell<T, D>& g = cells_[idx];
+        if (g.m > T(0)) {
+            for (int a = 0; a < D; ++a) g.v[a] = g.v[a] / g.m;
+            g.v[1] = g.v[1] + dt_ * gravity_;
+            for (int a = 0; a < D; ++a) g.v[a] = std::clamp(g.v[a], -allowed, allowed);
+        }
+        for (int a = 0; a < D; ++a) {
+            if ((n[a] < kBoundary && g.v[a] < T(0)) || (n[a] >= (res_ + 1) - kBoundary && g.v[a] > T(0))) {
+                for (int b = 0; b < D; ++b) g.v[b] = T(0);
+                g.m = T(0);
             }
         }
     }
 
+    // The same step on `threads` OpenMP threads (the multi-core CPU baseline; the reference itself
+    // is serial): P2G into one private grid per thread over a static split of the particles, the
+    // private grids summed per node in thread order, then the grid update and G2P split over nodes
+    // and particles.  Only P2G's summation order differs from advance().
+    void advance_parallel(int threads) {
+        const int64_t nn = num_nodes();
+        const int64_t np = int64_t(particles_.size());
+        std::vector<std::vector<Cell<T, D>>> priv(size_t(std::max(threads, 1)));
+        std::exception_ptr err = nullptr;
+        cells_.resize(size_t(nn));
+#pragma omp parallel num_threads(threads)
+        {
+            int t = 0, nt = 1;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+            nt = omp_get_num_threads();
+#endif
+            try {
+                auto& g = priv[size_t(t)];
+                zero_grid(g);
+                const int64_t a = np * t / nt, b = np * (t + 1) / nt;
+                for (int64_t pp = a; pp < b; ++pp) deposit(particles_[size_t(pp)], g);
+            } catch (...) {
+#pragma omp critical
+                if (!err) err = std::current_exception();
+            }
+#pragma omp barrier
+#pragma omp for schedule(static)
+            for (int64_t i = 0; i < nn; ++i) {
+                Cell<T, D> c = priv[0][size_t(i)];
+                for (int k = 1; k < nt; ++k) {
+                    const Cell<T, D>& o = priv[size_t(k)][size_t(i)];
+                    for (int a = 0; a < D; ++a) c.v[a] = c.v[a] + o.v[a];
+                    c.m = c.m + o.m;
+                }
+                cells_[size_t(i)] = c;
+                update_node(i);
+            }
+#pragma omp for schedule(static)
+            for (int64_t pp = 0; pp < np; ++pp) {
+                try {
+                    update_particle(particles_[size_t(pp)]);
+                } catch (...) {
+#pragma omp critical
+                    if (!err) err = std::current_exception();
+                }
+            }
+        }
+        if (err) std::rethrow_exception(err);
+    }
+
     // nclr.h:167-261
     void g2p() {
-        for (size_t pp = 0; pp < particles_.size(); ++pp) {
-            auto& p = particles_[pp];
+        for (size_t pp = 0; pp < particles_.size(); ++pp) update_particle(particles_[pp]);
+    }
+
+    void update_particle(Particle<T, D>& p) const {
+        {
             int base[D];
             T fx[D], w[D][3];
             base_and_weights(p.x, base, fx, w);
@@ -458,7 +515,7 @@ class Simulation {
             for_each_offset([&](const int off[D]) {
                 T dpos[D];
                 for (int a = 0; a < D; ++a) dpos[a] = T(off[a]) - fx[a];
-                const Cell<T, D>& cell = cell_at(base, off);
+                const Cell<T, D>& cell = cell_in(cells_, base, off);
                 T weight = w[0][off[0]];
                 for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
                 for (int a = 0; a < D; ++a) p.v[a] = p.v[a] + weight * cell.v[a];
@@ -584,14 +641,20 @@ class Simulation {
     // Flattened x-major index, checked only on the flat range like
     // std::vector::at (nclr.h:141-142,152,163-164): per-axis overflow on the
     // faster axes aliases into a neighbouring row, as in the reference.
-    Cell<T, D>& cell_at(const int base[D], const int off[D]) {
+    template <typename Grid>
+    auto cell_in(Grid& grid, const int base[D], const int off[D]) const -> decltype(grid[0]) {
         int64_t idx = 0;
         for (int a = 0; a < D; ++a) idx = idx * (res_ + 1) + int64_t(base[a] + off[a]);
-        if (idx < 0 || idx >= int64_t(cells_.size())) throw OutOfRange("grid index out of range");
-        return cells_[size_t(idx)];
+        if (idx < 0 || idx >= int64_t(grid.size())) throw OutOfRange("grid index out of range");
+        return grid[size_t(idx)];
     }
-    const Cell<T, D>& cell_at(const int base[D], const int off[D]) const {
-        return const_cast<Simulation*>(this)->cell_at(base, off);
+
+    void zero_grid(std::vector<Cell<T, D>>& grid) const {
+        grid.assign(size_t(num_nodes()), Cell<T, D>{});
+        for (auto& c : grid) {
+            for (int a = 0; a < D; ++a) c.v[a] = T(0);
+            c.m = T(0);
+        }
     }
 
     static void copy(const T a[D][D], T b[D][D]) {

@@ -204,18 +204,35 @@ def linspaced(n: int, low: float, high: float, fp64: bool = False) -> np.ndarray
 
 
 def time_steps(state: State, material: int, res: int, dt: float = 1e-4, E: float = 1e4, nu: float = 0.2,
-               gravity: float = -100.0, warmup: int = 1, steps: int = 1, identity_mode: int = 0) -> float:
-    """Seconds for `steps` serial FP32 steps of the -O3 -ffast-math oracle build (the CPU baseline)."""
+               gravity: float = -100.0, warmup: int = 1, steps: int = 1, identity_mode: int = 0,
+               threads: int = 1) -> float:
+    """Seconds for `steps` FP32 steps of the -O3 -ffast-math oracle build (the CPU baseline), serial
+    (threads = 1, as the reference runs) or on `threads` OpenMP threads."""
     s = state.astype(np.float32)
     lib = _lib(fast=True)
     secs = lib.oracle_time_steps_f32(
         ctypes.c_int(s.dim), ctypes.c_int(material), ctypes.c_int(res), ctypes.c_float(dt), ctypes.c_float(E),
         ctypes.c_float(nu), ctypes.c_float(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(s.n),
         ctypes.c_int(warmup), ctypes.c_int(steps), _ptr(s.x), _ptr(s.v), _ptr(s.F), _ptr(s.C), _ptr(s.Jp),
-        _ptr(s.mass), _ptr(s.volume))
+        _ptr(s.mass), _ptr(s.volume), ctypes.c_int(threads))
     if secs < 0:
         raise RuntimeError(lib.oracle_last_error().decode())
     return secs
+
+
+def run_threads(state: State, material: int, res: int = 64, dt: float = 1e-4, E: float = 1e4, nu: float = 0.2,
+                gravity: float = -100.0, steps: int = 1, identity_mode: int = 0, threads: int = 1,
+                fast: bool = True) -> State:
+    """FP32 steps of the oracle on `threads` OpenMP threads (the fast build has OpenMP)."""
+    s = state.astype(np.float32)
+    lib = _lib(fast=fast)
+    rc = lib.oracle_run_threads_f32(
+        ctypes.c_int(s.dim), ctypes.c_int(material), ctypes.c_int(res), ctypes.c_float(dt), ctypes.c_float(E),
+        ctypes.c_float(nu), ctypes.c_float(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(s.n),
+        ctypes.c_int(steps), ctypes.c_int(threads), _ptr(s.x), _ptr(s.v), _ptr(s.F), _ptr(s.C), _ptr(s.Jp),
+        _ptr(s.mass), _ptr(s.volume))
+    _check(rc, lib)
+    return s
 
 
 @dataclass

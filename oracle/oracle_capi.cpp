@@ -86,10 +86,15 @@ int guarded(Fn&& fn) {
 
 template <typename T, int D>
 int run(int material, int res, T dt, T E, T nu, T g, int idm, int64_t n, int nsteps, T* x, T* v, T* F, T* C, T* Jp,
-        const T* mass, const T* volume, T* grid_v, T* grid_m) {
+        const T* mass, const T* volume, T* grid_v, T* grid_m, int threads = 1) {
     return guarded([&] {
         oracle::Simulation<T, D> sim(load<T, D>(n, x, v, F, C, Jp, mass, volume), material, res, dt, E, nu, g, idm);
-        for (int s = 0; s < nsteps; ++s) sim.advance();
+        for (int s = 0; s < nsteps; ++s) {
+            if (threads > 1)
+                sim.advance_parallel(threads);
+            else
+                sim.advance();
+        }
         store<T, D>(sim.particles(), x, v, F, C, Jp);
         store_grid<T, D>(sim.grid(), grid_v, grid_m);
     });
@@ -302,17 +307,34 @@ int oracle_polar_vjp_f64(int dim, int64_t n, const double* A, const double* Rbar
 // Times `steps` calls of advance() after `warmup` untimed ones on an FP32
 // simulation built from the given particles (the reference's serial CPU
 // path, restated).  Returns seconds, or a negative value on error.
+// FP32 steps of the oracle on `threads` OpenMP threads (1: the serial step the reference runs).
+int oracle_run_threads_f32(int dim, int material, int res, float dt, float E, float nu, float g, int idm, int64_t n,
+                           int nsteps, int threads, float* x, float* v, float* F, float* C, float* Jp,
+                           const float* mass, const float* volume) {
+    return DIM_DISPATCH(dim,
+                        (run<float, 2>(material, res, dt, E, nu, g, idm, n, nsteps, x, v, F, C, Jp, mass, volume,
+                                       nullptr, nullptr, threads)),
+                        (run<float, 3>(material, res, dt, E, nu, g, idm, n, nsteps, x, v, F, C, Jp, mass, volume,
+                                       nullptr, nullptr, threads)));
+}
+
 double oracle_time_steps_f32(int dim, int material, int res, float dt, float E, float nu, float g, int idm,
                              int64_t n, int warmup, int steps, const float* x, const float* v, const float* F,
-                             const float* C, const float* Jp, const float* mass, const float* volume) {
+                             const float* C, const float* Jp, const float* mass, const float* volume, int threads) {
     double secs = -1.0;
     auto body = [&](auto dim_tag) {
         constexpr int D = decltype(dim_tag)::value;
         oracle::Simulation<float, D> sim(load<float, D>(n, x, v, F, C, Jp, mass, volume), material, res, dt, E, nu,
                                          g, idm);
-        for (int s = 0; s < warmup; ++s) sim.advance();
+        auto step = [&] {
+            if (threads > 1)
+                sim.advance_parallel(threads);
+            else
+                sim.advance();
+        };
+        for (int s = 0; s < warmup; ++s) step();
         const auto t0 = std::chrono::steady_clock::now();
-        for (int s = 0; s < steps; ++s) sim.advance();
+        for (int s = 0; s < steps; ++s) step();
         const auto t1 = std::chrono::steady_clock::now();
         secs = std::chrono::duration<double>(t1 - t0).count();
     };

@@ -259,3 +259,26 @@ def test_fp32_tracks_fp64_over_a_short_run():
     b = O.run(s, O.SNOW, res=64, steps=20, fp64=True)
     assert np.abs(a.x - b.x).max() < 1e-6
     assert np.abs(a.F - b.F).max() < 1e-5
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_parallel_step_matches_serial(dim):
+    """The OpenMP step of the CPU baseline differs from the serial one only in P2G's summation
+    order: after 3 steps every field agrees to FP32 rounding (the one-step parity metric, 1e-5)."""
+    from parity import assert_parity
+
+    side = 20 if dim == 2 else 8
+    g = np.linspace(0.3, 0.45, side, dtype=np.float32)
+    a = np.stack(np.meshgrid(*([g] * dim), indexing="ij"), -1).reshape(-1, dim)
+    x = np.concatenate([a, a + np.float32(0.2)]).astype(np.float32)
+    n = len(x)
+    v = np.zeros_like(x)
+    v[: n // 2, 0] = 2.0
+    v[n // 2:, 0] = -2.0
+    F = np.broadcast_to(np.eye(dim, dtype=np.float32), (n, dim, dim)).copy()
+    st = O.State(x=x, v=v, F=F, C=np.zeros_like(F), Jp=np.ones(n, np.float32), mass=np.ones(n, np.float32),
+                 volume=np.ones(n, np.float32))
+    for mat in (O.SNOW, O.JELLY, O.LIQUID):
+        ser = O.run_threads(st, mat, res=32, steps=3, identity_mode=1, threads=1)
+        par = O.run_threads(st, mat, res=32, steps=3, identity_mode=1, threads=4)
+        assert_parity(par, ser, inv_dx=32.0, label=f"OpenMP oracle, material {mat}")

@@ -182,17 +182,36 @@ def test_kernel_timing_and_params():
             sim.kernel_time("nope")
 
 
-def test_drift_against_fp64():
-    """Reports the RMS position drift of the engine and of the FP32 oracle against the FP64
-    oracle over 300 steps of the 2D snow scene; the engine's drift must stay within a small
-    factor of the FP32 oracle's own."""
-    sc = scenes.two_cubes_2d(MaterialModel.kSnow)
-    steps = 300
-    g = gpu_state(sc.particles, sc.material, sc.res, steps)
-    r32 = O.run(to_oracle(sc.particles), 0, res=sc.res, steps=steps)
-    r64 = O.run(to_oracle(sc.particles), 0, res=sc.res, steps=steps, fp64=True)
-    rms_gpu = float(np.sqrt(((g.x - r64.x) ** 2).sum(1).mean()))
-    rms_32 = float(np.sqrt(((r32.x - r64.x) ** 2).sum(1).mean()))
-    print(f"[drift] {steps} steps: rms |x_gpu - x_fp64| = {rms_gpu:.3e}, rms |x_fp32oracle - x_fp64| = {rms_32:.3e}",
-          flush=True)
-    assert rms_gpu <= 10 * rms_32 + 1e-6
+def drift_curve(p: Particles, mat, res, checkpoints, identity_mode=0):
+    """RMS position error against the FP64 oracle of the engine and of the FP32 oracle, both
+    advanced from the same state, at each checkpoint step."""
+    rows = []
+    with MPMSimulation(p, mat, res=res, identity_mode=identity_mode) as sim:
+        r32 = r64 = to_oracle(p)
+        done = 0
+        for c in checkpoints:
+            k = c - done
+            sim.advance(k)
+            g = sim.particles()
+            r32 = O.run(r32, int(mat), res=res, steps=k, identity_mode=identity_mode)
+            r64 = O.run(r64, int(mat), res=res, steps=k, identity_mode=identity_mode, fp64=True)
+            done = c
+            rms = lambda a: float(np.sqrt(((np.asarray(a, np.float64) - r64.x) ** 2).sum(1).mean()))
+            rows.append((c, rms(g.x), rms(r32.x)))
+    return rows
+
+
+@pytest.mark.parametrize("case", ["two_cubes_2d_snow", "collision_3d_jelly"])
+def test_drift_against_fp64(case):
+    """The trajectory drift against the FP64 oracle: the engine's RMS position error must stay
+    within a small factor of the FP32 oracle's own, at every checkpoint (printed as a curve)."""
+    if case == "two_cubes_2d_snow":
+        sc = scenes.two_cubes_2d(MaterialModel.kSnow)
+        rows = drift_curve(sc.particles, sc.material, sc.res, [1, 10, 100, 300, 1000])
+    else:
+        rows = drift_curve(collision_scene_3d(side_res=10, identity_mode=1), MaterialModel.kJelly, 32,
+                           [1, 10, 50, 100, 200], identity_mode=1)
+    for step, rms_gpu, rms_32 in rows:
+        print(f"[drift] {case} step {step}: rms |x_gpu - x_fp64| = {rms_gpu:.3e}, "
+              f"rms |x_fp32oracle - x_fp64| = {rms_32:.3e}", flush=True)
+        assert rms_gpu <= 10 * rms_32 + 1e-6, (step, rms_gpu, rms_32)
