@@ -1,8 +1,8 @@
 // Test infrastructure only: reverse-mode adjoint of one step of the CPU oracle (nclr_oracle.hpp).
 //
 // The reference has no derivatives; this is the FP64 CPU adjoint the GPU adjoint kernels are
-// checked against.  It differentiates the restated step (src/nclr.h:104-310) for the jelly
-// material (F' = (I' + dt C') F, src/nclr.h:230-234), holding mass, volume and Jp fixed:
+// checked against.  It differentiates the restated step (src/nclr.h:104-310) for every material,
+// holding mass and volume fixed (shown for jelly, F' = (I' + dt C') F, src/nclr.h:230-234):
 //
 //   P2G   mom_N = sum_p w (m v + A dpos),  mass_N = sum_p w m,
 //         A = -dt V0 4 inv_dx^2 [2 mu (F - R) F^T + lambda (J-1) J 1] + m C,  R = polar(F)
@@ -14,6 +14,15 @@
 // derivative uses R^T dR = Y skew with S Y + Y S = skew(R^T dF) - ..., i.e. for a cotangent
 // Rbar: Fbar += 2 R Y, S Y + Y S = skew(R^T Rbar), S = R^T F.
 //
+// Snow (src/nclr.h:235-251): F^ = (I' + dt C') F = U S V^T (nclr_svd, signed S), F' = U f(S) V^T with
+// f = clamp(., 1 - 2.5e-2, 1 + 4.5e-3), Jp' = clamp(Jp det F^ / det F', 0.6, 20), and the P2G
+// hardening mu = mu0 e, lambda = lambda0 e, e = exp(10 (1 - Jp)).  The backward pass of
+// U f(S) V^T: with P = U^T Fbar' V, the cotangent of F^ is U Q V^T, Q_ii = f'(s_i) P_ii and, for
+// i != j, Q_ij = a_ij (P_ij + P_ji) / 2 + b_ij (P_ij - P_ji) / 2 with the divided differences
+// a_ij = (f_i - f_j) / (s_i - s_j) (f'(s_i) when s_i = s_j) and b_ij = (f_i + f_j) / (s_i + s_j).
+// Liquid (src/nclr.h:252-258): F' = diag<dim>(1) with F'_00 = J = prod S (= |det F^| in 2D, where
+// nclr_svd's singular values are non-negative, det F^ in 3D), so Fbar^ = Fbar'_00 dJ/dF^.
+//
 // Pinned by central finite differences of the FP64 oracle step (tests/test_oracle_adjoint.py).
 #pragma once
 
@@ -23,11 +32,48 @@ namespace oracle {
 
 template <typename T, int D>
 struct StepGrads {
-    std::vector<T> x, v, F, C;  // particle-major: x[n][D], F[n][D][D]
+    std::vector<T> x, v, F, C, Jp;  // particle-major: x[n][D], F[n][D][D], Jp[n]
     T E = T(0);
     T E_abs = T(0);  // sum over particles of |their contribution to E|: the scale of E's rounding
-    explicit StepGrads(size_t n = 0) : x(n * D, T(0)), v(n * D, T(0)), F(n * D * D, T(0)), C(n * D * D, T(0)) {}
+    explicit StepGrads(size_t n = 0)
+        : x(n * D, T(0)), v(n * D, T(0)), F(n * D * D, T(0)), C(n * D * D, T(0)), Jp(n, T(0)) {}
 };
+
+// Fbar += the cotangent of A through A -> U f(S) V^T (the snow clamp of the singular values).
+template <typename T, int D>
+inline void clamp_svd_backward(const T U[D][D], const T sig[D], const T V[D][D], T lo, T hi, const T Gbar[D][D],
+                               T Fbar[D][D]) {
+    T P[D][D], Q[D][D], f[D], fp[D];
+    for (int i = 0; i < D; ++i) {
+        f[i] = std::clamp(sig[i], lo, hi);
+        fp[i] = (sig[i] > lo && sig[i] < hi) ? T(1) : T(0);
+    }
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            T s = T(0);
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) s += U[a][i] * Gbar[a][b] * V[b][j];
+            P[i][j] = s;
+        }
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            if (i == j) {
+                Q[i][i] = fp[i] * P[i][i];
+                continue;
+            }
+            const T ds = sig[i] - sig[j];
+            const T a = ds != T(0) ? (f[i] - f[j]) / ds : fp[i];
+            const T b = (f[i] + f[j]) / (sig[i] + sig[j]);
+            Q[i][j] = a * T(0.5) * (P[i][j] + P[j][i]) + b * T(0.5) * (P[i][j] - P[j][i]);
+        }
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b) {
+            T s = T(0);
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) s += U[a][i] * Q[i][j] * V[b][j];
+            Fbar[a][b] += s;
+        }
+}
 
 // Quadratic B-spline weight derivatives d w_k / d fx.
 template <typename T>
@@ -106,14 +152,14 @@ inline void polar_backward(const T F[D][D], const T R[D][D], const T Rbar[D][D],
         }
 }
 
-// One-step vector-Jacobian product for a jelly simulation: given the cotangents `out` of the
-// state after the step, returns the cotangents `in` of the state before it and of E.
+// One-step vector-Jacobian product: given the cotangents `out` of the state after the step,
+// returns the cotangents `in` of the state before it and of E.
 template <typename T, int D>
-void adjoint_step_jelly(const std::vector<Particle<T, D>>& ps, int res, T dt, T E, T nu, T gravity, int idm,
-                        const StepGrads<T, D>& out, StepGrads<T, D>& in) {
+void adjoint_step(const std::vector<Particle<T, D>>& ps, int material, int res, T dt, T E, T nu, T gravity, int idm,
+                  const StepGrads<T, D>& out, StepGrads<T, D>& in) {
     const size_t n = ps.size();
     in = StepGrads<T, D>(n);
-    Simulation<T, D> sim(ps, kJelly, res, dt, E, nu, gravity, idm);
+    Simulation<T, D> sim(ps, material, res, dt, E, nu, gravity, idm);
     sim.p2g();
     const std::vector<Cell<T, D>> pre = sim.grid();  // momentum, mass
     sim.grid_op();
@@ -144,6 +190,46 @@ void adjoint_step_jelly(const std::vector<Particle<T, D>>& ps, int res, T dt, T 
             for (int a = 0; a < D; ++a)
                 for (int b = 0; b < D; ++b) Cn[a][b] += four_inv_dx * weight * g.v[a] * (T(off[b]) - fx[b]);
         });
+        // F^ = (I' + dt C') F and the cotangent of F^ from those of F' and Jp'
+        T Mh[D][D], Fh[D][D], Fhbar[D][D];
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) Mh[a][b] = Iprime[a][b] + dt * Cn[a][b];
+        matmul<T, D>(Mh, P.F, Fh);
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) Fhbar[a][b] = material == kJelly ? out.F[(p * D + a) * D + b] : T(0);
+        in.Jp[p] += material == kSnow ? T(0) : out.Jp[p];  // jelly and liquid keep Jp
+        if (material == kLiquid) {
+            T U[D][D], sig[D], V[D][D], cof[D][D];
+            nclr_svd<T, D>(Fh, U, sig, V);
+            T J = T(1);
+            for (int d = 0; d < D; ++d) J *= sig[d];
+            cofactor<T>(Fh, cof);
+            const T sgn = J * det<T>(Fh) < T(0) ? T(-1) : T(1);
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) Fhbar[a][b] = out.F[(p * D) * D] * sgn * cof[a][b];
+        } else if (material == kSnow) {
+            T U[D][D], sig[D], V[D][D];
+            nclr_svd<T, D>(Fh, U, sig, V);
+            const T lo = T(float(1.0 - 2.5e-2));
+            const T hi = T(float(1.0 + 4.5e-3));
+            T US[D][D], Fn[D][D];
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) US[a][b] = U[a][b] * std::clamp(sig[b], lo, hi);
+            matmul_bt<T, D>(US, V, Fn);
+            const T old_J = det<T>(Fh), new_J = det<T>(Fn);
+            const T r = P.Jp * old_J / new_J;
+            const T rbar = (r > T(0.6f) && r < T(20.0f)) ? out.Jp[p] : T(0);
+            in.Jp[p] += rbar * old_J / new_J;
+            T cof_h[D][D], cof_n[D][D], Gn[D][D];
+            cofactor<T>(Fh, cof_h);
+            cofactor<T>(Fn, cof_n);
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) {
+                    Fhbar[a][b] += rbar * P.Jp / new_J * cof_h[a][b];
+                    Gn[a][b] = out.F[(p * D + a) * D + b] - rbar * P.Jp * old_J / (new_J * new_J) * cof_n[a][b];
+                }
+            clamp_svd_backward<T, D>(U, sig, V, lo, hi, Gn, Fhbar);
+        }
         T vbar_t[D], Cbar_t[D][D];
         for (int a = 0; a < D; ++a) {
             vbar_t[a] = out.v[p * D + a] + dt * out.x[p * D + a];
@@ -153,8 +239,8 @@ void adjoint_step_jelly(const std::vector<Particle<T, D>>& ps, int res, T dt, T 
             for (int b = 0; b < D; ++b) {
                 T s = T(0), fb = T(0);
                 for (int k = 0; k < D; ++k) {
-                    s += out.F[(p * D + a) * D + k] * P.F[b][k];                             // (Fbar' F^T)_ab
-                    fb += (Iprime[k][a] + dt * Cn[k][a]) * out.F[(p * D + k) * D + b];      // (M^T Fbar')_ab
+                    s += Fhbar[a][k] * P.F[b][k];               // (Fbar^ F^T)_ab
+                    fb += Mh[k][a] * Fhbar[k][b];               // (M^T Fbar^)_ab
                 }
                 Cbar_t[a][b] = out.C[(p * D + a) * D + b] + dt * s;
                 in.F[(p * D + a) * D + b] += fb;
@@ -293,7 +379,10 @@ void adjoint_step_jelly(const std::vector<Particle<T, D>>& ps, int res, T dt, T 
         for (int a = 0; a < D; ++a)
             for (int b = 0; b < D; ++b) in.F[(p * D + a) * D + b] += Fbar[a][b] + Jbar * cof[a][b];
         const T lambar = sG * (J - T(1)) * J;
-        const T e = T(0.3f);
+        const T e = material == kSnow ? T(std::exp(10.0 * (1.0 - double(P.Jp))))
+                                      : (material == kJelly ? T(0.3f) : T(1.0f));
+        // snow: mu = mu0 e, lambda = lambda0 e with de/dJp = -10 e
+        if (material == kSnow) in.Jp[p] += (mubar * sim.mu_0 + lambar * sim.lambda_0) * (T(-10) * e);
         const T ep = e * (mubar / (T(2) * (T(1) + nu)) + lambar * nu / ((T(1) + nu) * (T(1) - T(2) * nu)));
         in.E += ep;
         in.E_abs += ep < T(0) ? -ep : ep;

@@ -244,43 +244,59 @@ class Grads:
     C: np.ndarray
     E: float = 0.0
     E_abs: float = 0.0  # sum of the particles' |contributions| to E (the scale of its rounding)
+    Jp: np.ndarray = None  # None = zero
+
+
+def adjoint(state: State, grads_out: Grads, material: int, res: int = 64, dt: float = 1e-4, E: float = 1e4,
+            nu: float = 0.2, gravity: float = -100.0, identity_mode: int = 0) -> Grads:
+    """FP64 vector-Jacobian product of one step of any material (nclr_adjoint.hpp): given dL/d(state
+    after the step) for x, v, F, C and Jp, returns dL/d(state before the step) and dL/dE."""
+    s = state.astype(np.float64)
+    n, d = s.n, s.dim
+    g = [np.ascontiguousarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")]
+    gj = np.zeros(n) if grads_out.Jp is None else np.ascontiguousarray(grads_out.Jp, dtype=np.float64)
+    outs = [np.zeros_like(a) for a in g]
+    oj = np.zeros(n)
+    e = np.zeros(2, np.float64)
+    lib = _lib()
+    rc = lib.oracle_adjoint_f64(
+        ctypes.c_int(d), ctypes.c_int(material), ctypes.c_int(res), ctypes.c_double(dt), ctypes.c_double(E),
+        ctypes.c_double(nu), ctypes.c_double(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(n), _ptr(s.x),
+        _ptr(s.v), _ptr(s.F), _ptr(s.C), _ptr(s.Jp), _ptr(s.mass), _ptr(s.volume), *(_ptr(a) for a in g), _ptr(gj),
+        *(_ptr(a) for a in outs), _ptr(oj), _ptr(e))
+    _check(rc, lib)
+    return Grads(*outs, E=float(e[0]), E_abs=float(e[1]), Jp=oj)
 
 
 def adjoint_jelly(state: State, grads_out: Grads, res: int = 64, dt: float = 1e-4, E: float = 1e4, nu: float = 0.2,
                   gravity: float = -100.0, identity_mode: int = 0) -> Grads:
-    """FP64 vector-Jacobian product of one jelly step (nclr_adjoint.hpp): given dL/d(state after the
-    step), returns dL/d(state before the step) and dL/dE."""
-    s = state.astype(np.float64)
-    n, d = s.n, s.dim
-    g = [np.ascontiguousarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")]
-    outs = [np.zeros_like(a) for a in g]
-    e = np.zeros(2, np.float64)
-    lib = _lib()
-    rc = lib.oracle_adjoint_jelly_f64(
-        ctypes.c_int(d), ctypes.c_int(res), ctypes.c_double(dt), ctypes.c_double(E), ctypes.c_double(nu),
-        ctypes.c_double(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(n), _ptr(s.x), _ptr(s.v), _ptr(s.F),
-        _ptr(s.C), _ptr(s.Jp), _ptr(s.mass), _ptr(s.volume), *(_ptr(a) for a in g), *(_ptr(a) for a in outs),
-        _ptr(e))
-    _check(rc, lib)
-    return Grads(*outs, E=float(e[0]), E_abs=float(e[1]))
+    """adjoint() for the jelly material."""
+    return adjoint(state, grads_out, JELLY, res, dt, E, nu, gravity, identity_mode)
 
 
-def rollout_vjp_jelly(state: State, grads_out: Grads, steps: int, res: int = 64, dt: float = 1e-4, E: float = 1e4,
-                      nu: float = 0.2, gravity: float = -100.0, identity_mode: int = 0) -> Grads:
-    """FP64 vector-Jacobian product of `steps` jelly steps: the FP64 trajectory is stored and the
+def rollout_vjp(state: State, grads_out: Grads, steps: int, material: int = JELLY, res: int = 64, dt: float = 1e-4,
+                E: float = 1e4, nu: float = 0.2, gravity: float = -100.0, identity_mode: int = 0) -> Grads:
+    """FP64 vector-Jacobian product of `steps` steps: the FP64 trajectory is stored and the
     one-step adjoint is chained backwards through it; dL/dE sums over the steps."""
     kw = dict(res=res, dt=dt, E=E, nu=nu, gravity=gravity, identity_mode=identity_mode)
     states = [state.astype(np.float64)]
     for _ in range(steps - 1):
-        states.append(run(states[-1], JELLY, steps=1, fp64=True, **kw))
-    g = Grads(*(np.asarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")))
+        states.append(run(states[-1], material, steps=1, fp64=True, **kw))
+    g = Grads(*(np.asarray(getattr(grads_out, k), dtype=np.float64) for k in ("x", "v", "F", "C")),
+              Jp=None if grads_out.Jp is None else np.asarray(grads_out.Jp, dtype=np.float64))
     total_E = total_abs = 0.0
     for s in reversed(states):
-        g = adjoint_jelly(s, g, **kw)
+        g = adjoint(s, g, material, **kw)
         total_E += g.E
         total_abs += g.E_abs
     g.E, g.E_abs = total_E, total_abs
     return g
+
+
+def rollout_vjp_jelly(state: State, grads_out: Grads, steps: int, res: int = 64, dt: float = 1e-4, E: float = 1e4,
+                      nu: float = 0.2, gravity: float = -100.0, identity_mode: int = 0) -> Grads:
+    """rollout_vjp() for the jelly material."""
+    return rollout_vjp(state, grads_out, steps, JELLY, res, dt, E, nu, gravity, identity_mode)
 
 
 def polar_vjp(A: np.ndarray, Rbar: np.ndarray) -> np.ndarray:

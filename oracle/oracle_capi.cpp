@@ -195,10 +195,10 @@ void stress_batch(int material, int res, T dt, T E, T nu, int64_t n, const T* F,
 }
 
 template <int D>
-int adjoint(int res, double dt, double E, double nu, double g, int idm, int64_t n, const double* x,
+int adjoint(int material, int res, double dt, double E, double nu, double g, int idm, int64_t n, const double* x,
             const double* v, const double* F, const double* C, const double* Jp, const double* mass,
             const double* volume, const double* gx, const double* gv, const double* gF, const double* gC,
-            double* ix, double* iv, double* iF, double* iC, double* iE) {
+            const double* gJ, double* ix, double* iv, double* iF, double* iC, double* iJ, double* iE) {
     return guarded([&] {
         const auto ps = load<double, D>(n, x, v, F, C, Jp, mass, volume);
         oracle::StepGrads<double, D> out(static_cast<size_t>(n)), in;
@@ -206,11 +206,13 @@ int adjoint(int res, double dt, double E, double nu, double g, int idm, int64_t 
         std::copy(gv, gv + n * D, out.v.begin());
         std::copy(gF, gF + n * D * D, out.F.begin());
         std::copy(gC, gC + n * D * D, out.C.begin());
-        oracle::adjoint_step_jelly<double, D>(ps, res, dt, E, nu, g, idm, out, in);
+        if (gJ) std::copy(gJ, gJ + n, out.Jp.begin());
+        oracle::adjoint_step<double, D>(ps, material, res, dt, E, nu, g, idm, out, in);
         std::copy(in.x.begin(), in.x.end(), ix);
         std::copy(in.v.begin(), in.v.end(), iv);
         std::copy(in.F.begin(), in.F.end(), iF);
         std::copy(in.C.begin(), in.C.end(), iC);
+        if (iJ) std::copy(in.Jp.begin(), in.Jp.end(), iJ);
         iE[0] = in.E;
         iE[1] = in.E_abs;
     });
@@ -285,18 +287,19 @@ const char* oracle_last_error(void) { return g_err.c_str(); }
 DEFINE_FOR(float, f32)
 DEFINE_FOR(double, f64)
 
-// FP64 vector-Jacobian product of one jelly step (nclr_adjoint.hpp); iE[0] = dL/dE, iE[1] = the
-// sum of the particles' |contributions| to it.
-int oracle_adjoint_jelly_f64(int dim, int res, double dt, double E, double nu, double g, int idm, int64_t n,
-                             const double* x, const double* v, const double* F, const double* C, const double* Jp,
-                             const double* mass, const double* volume, const double* gx, const double* gv,
-                             const double* gF, const double* gC, double* ix, double* iv, double* iF, double* iC,
-                             double* iE) {
+// FP64 vector-Jacobian product of one step of any material (nclr_adjoint.hpp): cotangents of
+// x, v, F, C and Jp (gJ / iJ may be NULL for zero / not wanted); iE[0] = dL/dE, iE[1] = the sum
+// of the particles' |contributions| to it.
+int oracle_adjoint_f64(int dim, int material, int res, double dt, double E, double nu, double g, int idm, int64_t n,
+                       const double* x, const double* v, const double* F, const double* C, const double* Jp,
+                       const double* mass, const double* volume, const double* gx, const double* gv,
+                       const double* gF, const double* gC, const double* gJ, double* ix, double* iv, double* iF,
+                       double* iC, double* iJ, double* iE) {
     return DIM_DISPATCH(dim,
-                        (adjoint<2>(res, dt, E, nu, g, idm, n, x, v, F, C, Jp, mass, volume, gx, gv, gF, gC, ix, iv,
-                                    iF, iC, iE)),
-                        (adjoint<3>(res, dt, E, nu, g, idm, n, x, v, F, C, Jp, mass, volume, gx, gv, gF, gC, ix, iv,
-                                    iF, iC, iE)));
+                        (adjoint<2>(material, res, dt, E, nu, g, idm, n, x, v, F, C, Jp, mass, volume, gx, gv, gF,
+                                    gC, gJ, ix, iv, iF, iC, iJ, iE)),
+                        (adjoint<3>(material, res, dt, E, nu, g, idm, n, x, v, F, C, Jp, mass, volume, gx, gv, gF,
+                                    gC, gJ, ix, iv, iF, iC, iJ, iE)));
 }
 
 // FP64 cotangent of the polar rotation factor: Abar = d<Rbar, R(A)>/dA.

@@ -161,3 +161,83 @@ def test_chained_adjoint_matches_central_differences_over_many_steps():
     hE = 1e-3 * E0
     fdE = (L(s, E0 + hE) - L(s, E0 - hE)) / (2 * hE)
     assert abs(fdE - g.E) <= 1e-4 * abs(fdE), (fdE, g.E)
+
+
+# ---- snow and liquid: the SVD-clamp backward, Jp as a differentiable channel -----------------
+
+def material_state(d, side, seed, material):
+    """Particles whose F^ = (I' + dt C) F has singular values on both sides of the snow clamp
+    [0.975, 1.0045] (but none near it), Jp away from its clamp [0.6, 20]."""
+    s = jelly_state(d, side, seed, F_noise=0.0)
+    rng = np.random.default_rng(seed + 100)
+    n = s.n
+    for p in range(n):
+        # F = R diag(sig) Q with sig drawn away from the clamp bounds
+        sig = rng.choice([0.96, 0.985, 0.995, 1.0, 1.002, 1.02], size=d) + rng.uniform(-1e-3, 1e-3, d)
+        Rm, _ = np.linalg.qr(rng.normal(size=(d, d)))
+        Qm, _ = np.linalg.qr(rng.normal(size=(d, d)))
+        if np.linalg.det(Rm) < 0:
+            Rm[:, 0] *= -1
+        if np.linalg.det(Qm) < 0:
+            Qm[:, 0] *= -1
+        s.F[p] = Rm @ np.diag(sig) @ Qm
+    s.C = rng.normal(0, 0.5, (n, d, d))
+    s.Jp = rng.uniform(0.9, 1.1, n)
+    return s
+
+
+def loss_m(s, G, material, **kw):
+    o = O.run(s, material, fp64=True, steps=1, identity_mode=1, **kw)
+    return sum(float(np.sum(getattr(G, k) * getattr(o, k))) for k in ("x", "v", "F", "C", "Jp"))
+
+
+def cotangent_m(s, seed):
+    G = random_cotangent(s, seed)
+    G.Jp = np.random.default_rng(seed + 7).normal(size=s.n)
+    return G
+
+
+@pytest.mark.parametrize("material", [O.SNOW, O.LIQUID], ids=["snow", "liquid"])
+@pytest.mark.parametrize("d,side", [(2, 5), (3, 3)])
+@pytest.mark.parametrize("field", ["x", "v", "F", "C", "Jp"])
+def test_adjoint_snow_liquid_matches_central_differences(material, d, side, field):
+    s = material_state(d, side, seed=11, material=material)
+    G = cotangent_m(s, seed=12)
+    gi = O.adjoint(s, G, material, identity_mode=1, **KW)
+    rng = np.random.default_rng(13)
+    flat = getattr(s, field).reshape(-1)
+    fd, an = [], []
+    for i in rng.choice(flat.size, min(10, flat.size), replace=False):
+        h = 1e-6 * max(1.0, abs(flat[i]))
+        sp, sm = s.copy(), s.copy()
+        getattr(sp, field).reshape(-1)[i] += h
+        getattr(sm, field).reshape(-1)[i] -= h
+        fd.append((loss_m(sp, G, material, **KW) - loss_m(sm, G, material, **KW)) / (2 * h))
+        an.append(getattr(gi, field).reshape(-1)[i])
+    fd, an = np.array(fd), np.array(an)
+    err = np.abs(fd - an).max() / max(np.abs(fd).max(), 1e-12)
+    assert err < 1e-5, (field, err, fd, an)
+
+
+@pytest.mark.parametrize("material", [O.SNOW, O.LIQUID], ids=["snow", "liquid"])
+@pytest.mark.parametrize("d,side", [(2, 5), (3, 3)])
+def test_adjoint_snow_liquid_E(material, d, side):
+    s = material_state(d, side, seed=21, material=material)
+    G = cotangent_m(s, seed=22)
+    gi = O.adjoint(s, G, material, identity_mode=1, **KW)
+    hE = 1e-2
+    fdE = (loss_m(s, G, material, **{**KW, "E": KW["E"] + hE})
+           - loss_m(s, G, material, **{**KW, "E": KW["E"] - hE})) / (2 * hE)
+    assert abs(fdE - gi.E) <= 1e-5 * abs(fdE) + 1e-9 * gi.E_abs, (fdE, gi.E)
+
+
+def test_snow_adjoint_sees_clamped_and_free_singular_values():
+    """The test state exercises both branches of the clamp: some singular values of F^ are
+    inside [0.975, 1.0045] and some outside (checked on the state itself)."""
+    s = material_state(3, 3, seed=11, material=O.SNOW)
+    inside = outside = 0
+    for p in range(s.n):
+        sig = np.linalg.svd(s.F[p], compute_uv=False)
+        inside += int(((sig > 0.9755) & (sig < 1.0040)).sum())
+        outside += int(((sig < 0.9745) | (sig > 1.0050)).sum())
+    assert inside > 5 and outside > 5
