@@ -189,7 +189,7 @@ def fwd_bwd(which: str):
 
 def scene_generation():
     """The 3D snow workload's 1,048,576-particle state built on the device (lattices, velocities,
-    F, C: a 24 x n float image, 100.7 MB written) against the host scene generator plus the
+    F, C, Jp: a 25 x n float image, 104.9 MB written) against the host scene generator plus the
     upload of the same image."""
     import numpy as np
     import torch
@@ -213,7 +213,7 @@ def scene_generation():
         t0 = time.perf_counter()
         h = scenes.four_cubes_3d().particles
         n = h.x.shape[0]
-        host = np.concatenate([h.x.T, h.v.T, h.F.reshape(n, -1).T, h.C.reshape(n, -1).T])
+        host = np.concatenate([h.x.T, h.v.T, h.F.reshape(n, -1).T, h.C.reshape(n, -1).T, h.Jp[None]])
         up = torch.as_tensor(host, device="cuda")
         torch.cuda.synchronize()
         host_ms.append(1e3 * (time.perf_counter() - t0))

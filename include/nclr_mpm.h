@@ -169,8 +169,8 @@ int mpm_svd(int dim, int64_t n, const float* A, float* U, float* sigma, float* V
  * calls mpm_synchronize before it reads the outputs from another stream.
  *
  * State cotangents and device state images are structure-of-arrays over the n particles of the
- * handle: channel c of particle i at g[c*n + i], with the mpm_grad_channels(dim) = 2 dim + 2 dim^2
- * channels x[dim], v[dim], F[dim*dim], C[dim*dim] (matrices row-major).  "Particle order" is the
+ * handle: channel c of particle i at g[c*n + i], with the mpm_grad_channels(dim) = 2 dim + 2 dim^2 + 1
+ * channels x[dim], v[dim], F[dim*dim], C[dim*dim] (matrices row-major), Jp.  "Particle order" is the
  * order of the arrays given to mpm_create; "slot order" is the engine's current storage order
  * (particles are re-sorted by tile every step). */
 int mpm_grad_channels(int dim);
@@ -182,7 +182,7 @@ int64_t mpm_state_bytes(mpm_handle h);
 int mpm_save_state(mpm_handle h, void* dst);
 int mpm_load_state(mpm_handle h, const void* src);
 
-/* Device image of x, v, F, C in particle order (get), or sets them from one (set; Jp, mass and
+/* Device image of x, v, F, C, Jp in particle order (get), or sets them from one (set; mass and
  * volume are kept and the slot order becomes the particle order). */
 int mpm_get_state_device(mpm_handle h, float* dst);
 int mpm_set_state_device(mpm_handle h, const float* src);
@@ -191,8 +191,9 @@ int mpm_set_state_device(mpm_handle h, const float* src);
 int mpm_grad_from_particle_order(mpm_handle h, const float* src, float* dst);
 int mpm_grad_to_particle_order(mpm_handle h, const float* src, float* dst);
 
-/* Vector-Jacobian product of one step taken from the current state (jelly scenes only, else
- * MPM_ERR_UNSUPPORTED): grad_out is the cotangent of the state after the step in ITS slot order,
+/* Vector-Jacobian product of one step taken from the current state (every material: jelly; snow
+ * with the backward pass of its singular-value clamp, its Jp update and its hardening; liquid):
+ * grad_out is the cotangent of the state after the step in ITS slot order,
  * grad_in (must not alias grad_out) receives the cotangent of the current state in the current
  * slot order, and grad_E[s] += dL/dE of scene s (nscenes doubles; NULL: not accumulated).  The
  * state and the step counter are unchanged; the grid export afterwards shows this step's grid. */
@@ -212,8 +213,9 @@ int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu);
  *
  * The reference's scene helpers (src/nclr_math.h:22-39, 100-147) and pt_3d_to_2d (149-167) as
  * device kernels.  Generators write a state image (the layout mpm_set_state_device takes:
- * channel c of particle i at image[c*n + i], channels x[dim], v[dim], F[dim*dim], C[dim*dim]) with
- * v from the shape, F = diag<dim>(1) (diag(1, 1, 0) in 3D for identity_mode 0, I for 1) and C = 0.
+ * channel c of particle i at image[c*n + i], channels x[dim], v[dim], F[dim*dim], C[dim*dim], Jp) with
+ * v from the shape, F = diag<dim>(1) (diag(1, 1, 0) in 3D for identity_mode 0, I for 1), C = 0 and
+ * Jp = 1.
  * `image`, `pos` and `out` are device memory; the calls run on the device's null stream and return
  * when the device is done.  A state built this way enters an engine through mpm_create (any x,
  * e.g. zeros) followed by mpm_set_state_device. */

@@ -1,18 +1,19 @@
-"""Differentiable jelly rollouts on the HIP engine, as a torch.autograd.Function.
+"""Differentiable rollouts on the HIP engine (jelly, snow, liquid), as a torch.autograd.Function.
 
 The forward pass advances the engine `steps` steps from a given state, saving a device snapshot
 of the state every `checkpoint_every` steps.  The backward pass walks the segments in reverse: it
 loads a snapshot and calls the engine's segment adjoint (mpm_backward_segment), which recomputes
-the states inside the segment and runs the hand-written adjoint kernels (G2P, grid update, P2G
-with the polar-decomposition backward) step by step in reverse.  Gradients flow to the initial
-x, v, F, C of every particle and to Young's modulus of every scene.
+the states inside the segment and runs the hand-written adjoint kernels (G2P with the snow
+clamp's singular-value backward, grid update, P2G with the polar-decomposition backward and the
+snow hardening) step by step in reverse.  Gradients flow to the initial x, v, F, C, Jp of every
+particle and to Young's modulus of every scene.
 
 State tensors are (channels, n) float32 on the engine's device: rows x[d], v[d], F[d*d], C[d*d]
-(matrices row-major), particles in the order the simulation was created with.
+(matrices row-major), Jp, particles in the order the simulation was created with.
 """
 from __future__ import annotations
 
-from typing import Tuple
+from typing import Optional, Tuple
 
 import torch
 
@@ -21,24 +22,30 @@ from .sim import _Engine
 
 
 def channels(dim: int) -> int:
-    """Rows of a state tensor: x, v, F, C."""
-    return 2 * dim + 2 * dim * dim
+    """Rows of a state tensor: x, v, F, C, Jp."""
+    return 2 * dim + 2 * dim * dim + 1
 
 
-def pack(x: torch.Tensor, v: torch.Tensor, F: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
-    """(n,d), (n,d), (n,d,d), (n,d,d) -> (channels, n) contiguous float32."""
+def pack(x: torch.Tensor, v: torch.Tensor, F: torch.Tensor, C: torch.Tensor,
+         Jp: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """(n,d), (n,d), (n,d,d), (n,d,d)[, (n,)] -> (channels, n) contiguous float32 (Jp = 1 if omitted,
+    the Particle ctor's value)."""
     n = x.shape[0]
-    rows = torch.cat([x.reshape(n, -1), v.reshape(n, -1), F.reshape(n, -1), C.reshape(n, -1)], dim=1)
+    if Jp is None:
+        Jp = torch.ones(n, dtype=x.dtype, device=x.device)
+    rows = torch.cat([x.reshape(n, -1), v.reshape(n, -1), F.reshape(n, -1), C.reshape(n, -1), Jp.reshape(n, 1)],
+                     dim=1)
     return rows.t().contiguous().float()
 
 
-def unpack(state: torch.Tensor, dim: int) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
-    """(channels, n) -> views x (n,d), v (n,d), F (n,d,d), C (n,d,d)."""
+def unpack(state: torch.Tensor, dim: int) -> Tuple[torch.Tensor, ...]:
+    """(channels, n) -> views x (n,d), v (n,d), F (n,d,d), C (n,d,d), Jp (n,)."""
     d = dim
     t = state.t()
     n = t.shape[0]
-    return (t[:, :d], t[:, d:2 * d], t[:, 2 * d:2 * d + d * d].reshape(n, d, d),
-            t[:, 2 * d + d * d:].reshape(n, d, d))
+    c0 = 2 * d + d * d
+    return (t[:, :d], t[:, d:2 * d], t[:, 2 * d:c0].reshape(n, d, d), t[:, c0:c0 + d * d].reshape(n, d, d),
+            t[:, c0 + d * d])
 
 
 def _ready(t: torch.Tensor) -> None:
@@ -105,7 +112,7 @@ class _Rollout(torch.autograd.Function):
 
 
 def rollout(sim: _Engine, state0: torch.Tensor, E: torch.Tensor, steps: int, checkpoint_every: int = 32):
-    """Differentiable `steps`-step rollout of a jelly simulation or batch from `state0`
+    """Differentiable `steps`-step rollout of a simulation or batch (any materials) from `state0`
     ((channels, n) float32 on the engine's device) with Young's modulus E (one per scene).
     Returns the final state in the same layout.  The simulation is left at the rollout's final
     state after the forward pass and at its initial state after the backward pass."""

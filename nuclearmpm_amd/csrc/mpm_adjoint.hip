@@ -1,21 +1,27 @@
-// Reverse-mode (adjoint) kernels of one jelly MLS-MPM step for gfx950.
+// Reverse-mode (adjoint) kernels of one MLS-MPM step (jelly, snow, liquid) for gfx950.
 //
-// Given the cotangent of the state after a step (x', v', F', C') these kernels produce the
-// cotangent of the state before it (x, v, F, C) and of Young's modulus, per scene.  They run after
+// Given the cotangent of the state after a step (x', v', F', C', Jp') these kernels produce the
+// cotangent of the state before it (x, v, F, C, Jp) and of Young's modulus, per scene.  They run after
 // the forward sort, P2G and grid kernels have been recomputed for the state before the step, and
 // reuse that step's tiles, work items and P2G slots:
 //
 //   g2p_adjoint   one workgroup per P2G work item: per particle, the cotangents of the gathered
 //                 velocity and affine matrix are scattered back to the 3^D nodes (exact fixed-point
 //                 sums in LDS, as in P2G, stored per work item); the partial x and F cotangents
-//                 that G2P itself contributes go to grad_in
+//                 that G2P itself contributes go to grad_in.  Snow and liquid first recompute the
+//                 forward's F^ = (I' + dt C') F bit for bit and pass the cotangents of F' and Jp'
+//                 back through the material update: snow's F' = U clamp(S) V^T (the backward pass
+//                 of a function of the singular values) and Jp' = clamp(Jp det F^ / det F'),
+//                 liquid's F'_00 = prod S
 //   grid_adjoint  one workgroup per node tile: sums both slot sets, recomputes grid_op
 //                 (src/nclr.h:263-310) and maps the velocity cotangent to the cotangent of the
 //                 pre-update {momentum, mass} (clamped components and sticky nodes pass nothing)
 //   p2g_adjoint   per particle: gathers the grid cotangent through the P2G stencil and the fused
 //                 affine matrix (src/nclr.h:104-165, 313-337), including the backward pass of the
 //                 polar decomposition F = R S: for a cotangent Rbar of R, Fbar = 2 R Y with Y skew
-//                 and S Y + Y S = skew(R^T Rbar); completes grad_in and the per-particle dL/dE
+//                 and S Y + Y S = skew(R^T Rbar); snow's hardening e = exp(10 (1 - Jp)) adds the
+//                 Jp cotangent of mu = mu0 e and lambda = lambda0 e; completes grad_in and the
+//                 per-particle dL/dE
 //   e_sum         per scene, a fixed-order double sum of the dL/dE contributions
 //
 // The same restatement on the CPU in FP64 (the oracle's adjoint) is checked against central
@@ -114,12 +120,134 @@ NCLR_DI void polar_backward(const M<D>& F, const M<D>& R, const M<D>& Rbar, M<D>
         }
 }
 
+// Fbar += the cotangent of A through A -> U f(S) V^T, f = the snow clamp of the signed singular
+// values (src/nclr.h:241-245).  With P = U^T Gbar V: Q_ii = f'(s_i) P_ii and, for i != j,
+// Q_ij = a_ij (P_ij + P_ji) / 2 + b_ij (P_ij - P_ji) / 2 with the divided differences
+// a_ij = (f_i - f_j) / (s_i - s_j) (1 when both are inside the clamp, 0 when both are clamped to the
+// same bound) and b_ij = (f_i + f_j) / (s_i + s_j); then Fbar += U Q V^T.
+template <int D>
+NCLR_DI void clamp_svd_backward(const M<D>& U, const float (&sig)[D], const M<D>& V, float lo, float hi,
+                                const M<D>& Gbar, M<D>& Fbar) {
+    float f[D];
+    int side[D];  // -1: at or below lo, 1: at or above hi, 0: inside
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        side[i] = sig[i] <= lo ? -1 : (hi <= sig[i] ? 1 : 0);
+        f[i] = side[i] < 0 ? lo : (side[i] > 0 ? hi : sig[i]);
+    }
+    M<D> GV, P;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            float t = 0.0f;
+#pragma unroll
+            for (int b = 0; b < D; ++b) t += Gbar.a[a][b] * V.a[b][j];
+            GV.a[a][j] = t;
+        }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            float t = 0.0f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) t += U.a[a][i] * GV.a[a][j];
+            P.a[i][j] = t;
+        }
+    M<D> Q;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            if (i == j) {
+                Q.a[i][i] = side[i] == 0 ? P.a[i][i] : 0.0f;
+                continue;
+            }
+            const float a = (side[i] == 0 && side[j] == 0) ? 1.0f
+                            : (side[i] == side[j] ? 0.0f : (f[i] - f[j]) / (sig[i] - sig[j]));
+            const float b = (f[i] + f[j]) / (sig[i] + sig[j]);
+            Q.a[i][j] = 0.5f * (a * (P.a[i][j] + P.a[j][i]) + b * (P.a[i][j] - P.a[j][i]));
+        }
+    M<D> UQ;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            float t = 0.0f;
+#pragma unroll
+            for (int i = 0; i < D; ++i) t += U.a[a][i] * Q.a[i][j];
+            UQ.a[a][j] = t;
+        }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            float t = 0.0f;
+#pragma unroll
+            for (int j = 0; j < D; ++j) t += UQ.a[a][j] * V.a[b][j];
+            Fbar.a[a][b] += t;
+        }
+}
+
+// Cotangents of F^ = (I' + dt C') F and of the old Jp from those of the material update's outputs
+// (src/nclr.h:230-258): jelly F' = F^; snow F' = U clamp(S) V^T and Jp' = clamp(Jp det F^ / det F',
+// 0.6, 20); liquid F' = diag<dim>(1) with F'_00 = J = prod S (|det F^| in 2D, det F^ in 3D).  The SVD
+// is the forward's (same method and operation order).
+template <int D, bool EXACT>
+NCLR_DI void material_backward(int material, const M<D>& Fh, float Jp, const M<D>& Fbar_new, float Jbar_new,
+                               M<D>& Fhbar, float& Jbar) {
+    if (material == MPM_JELLY) {
+        Fhbar = Fbar_new;
+        Jbar = Jbar_new;
+        return;
+    }
+    M<D> U, V;
+    float sig[D];
+    step_svd<EXACT, D>(Fh, U, sig, V);
+    if (material == MPM_SNOW) {
+        const float lo = float(1.0 - 2.5e-2), hi = float(1.0 + 4.5e-3);
+        M<D> US;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) US.a[a][b] = U.a[a][b] * ((sig[b] < lo) ? lo : ((hi < sig[b]) ? hi : sig[b]));
+        const M<D> Fn = matmul_bt<D>(US, V);
+        const float old_J = det(Fh), new_J = det(Fn);
+        const float r = Jp * old_J / new_J;
+        const float rbar = (r > 0.6f && r < 20.0f) ? Jbar_new : 0.0f;
+        Jbar = rbar * old_J / new_J;
+        const M<D> cof_h = cofactor(Fh), cof_n = cofactor(Fn);
+        const float c1 = rbar * Jp / new_J, c2 = rbar * Jp * old_J / (new_J * new_J);
+        M<D> Gn;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                Fhbar.a[a][b] = c1 * cof_h.a[a][b];
+                Gn.a[a][b] = Fbar_new.a[a][b] - c2 * cof_n.a[a][b];
+            }
+        clamp_svd_backward<D>(U, sig, V, lo, hi, Gn, Fhbar);
+    } else {
+        double J = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) J *= double(sig[d]);
+        const float sgn = float(J) * det(Fh) < 0.0f ? -1.0f : 1.0f;
+        const M<D> cof = cofactor(Fh);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) Fhbar.a[a][b] = Fbar_new.a[0][0] * sgn * cof.a[a][b];
+        Jbar = Jbar_new;
+    }
+}
+
 NCLR_DI float ldg(const float* __restrict__ g, int64_t gstride, int c, int64_t j) { return g[int64_t(c) * gstride + j]; }
 
 // ---------------------------------------------------------------------------------------------
-// G2P adjoint (src/nclr.h:167-234, jelly)
+// G2P adjoint (src/nclr.h:167-258).  GENERAL: any material (the scene's, at run time); otherwise
+// every scene is jelly.
 // ---------------------------------------------------------------------------------------------
-template <int D, int PPT>
+template <int D, int PPT, bool GENERAL, bool EXACT>
 __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
                                                      const uint32_t* __restrict__ tile_start,
                                                      const uint32_t* __restrict__ tile_end,
@@ -187,13 +315,73 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
             vb[k][a] = ldg(gout, gstride, L::V + a, j) + sp.dt * ldg(gout, gstride, L::X + a, j);
             wsum *= fabsf(w[a][0]) + fabsf(w[a][1]) + fabsf(w[a][2]);
         }
+        // Fb: cotangent of F^ = (I' + dt C') F
+        M<D> Fb;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) Fb.a[a][b] = ldg(gout, gstride, L::F + a * D + b, j);
+        if constexpr (GENERAL) {
+            // the forward's C' (k_g2p_sorted's operation order, so F^ is the forward's bit for bit)
+            M<D> Cf;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) Cf.a[a][b] = 0.0f;
+#pragma unroll
+            for (int o = 0; o < K; ++o) {
+                int off[D], nidx[D];
+                float dpos[D];
+                offsets<D>(o, off);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    dpos[a] = float(off[a]) - fx[a];
+                    nidx[a] = base[a] + off[a];
+                }
+                const float4 g = gv[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
+                const float gvel[3] = {g.x, g.y, g.z};
+                float weight = w[0][off[0]];
+#pragma unroll
+                for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const float wv = four_inv_dx * (weight * gvel[a]);
+#pragma unroll
+                    for (int b = 0; b < D; ++b) Cf.a[a][b] = Cf.a[a][b] + wv * dpos[b];
+                }
+            }
+            M<D> Mx;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    const float id1 = (a == b && (a < 2 || sp.identity_mode == 1)) ? 1.0f : 0.0f;
+                    Mx.a[a][b] = id1 + sp.dt * Cf.a[a][b];
+                }
+            const M<D> Fh = matmul<D>(Mx, F);
+            const float Jp = ldf(st, p.stride, L::JP, i);
+            const M<D> Fbn = Fb;
+            float Jb;
+            material_backward<D, EXACT>(sp.material, Fh, Jp, Fbn, ldg(gout, gstride, L::JP, j), Fb, Jb);
+            // F cotangent (I' + dt C')^T Fbar^ and the Jp cotangent of the G2P update
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    float t = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < D; ++q) t += Mx.a[q][a] * Fb.a[q][b];
+                    gin[int64_t(L::F + a * D + b) * gstride + i] = t;
+                }
+            gin[int64_t(L::JP) * gstride + i] = Jb;
+        }
 #pragma unroll
         for (int a = 0; a < D; ++a)
 #pragma unroll
             for (int b = 0; b < D; ++b) {
                 float s = 0.0f;
 #pragma unroll
-                for (int q = 0; q < D; ++q) s += ldg(gout, gstride, L::F + a * D + q, j) * F.a[b][q];
+                for (int q = 0; q < D; ++q) s += Fb.a[a][q] * F.a[b][q];
                 Cb[k].a[a][b] = ldg(gout, gstride, L::C + a * D + b, j) + sp.dt * s;
             }
 #pragma unroll
@@ -298,19 +486,22 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
 #pragma unroll
         for (int a = 0; a < D; ++a)
             gin[int64_t(L::X + a) * gstride + i] = ldg(gout, gstride, L::X + a, j) + sp.inv_dx * fxbar[a];
-        // F' = (I' + dt C) F  =>  F cotangent (I' + dt C)^T F'
+        if constexpr (!GENERAL) {
+            // jelly: F' = (I' + dt C) F  =>  F cotangent (I' + dt C)^T F'; Jp is carried unchanged
 #pragma unroll
-        for (int a = 0; a < D; ++a)
+            for (int a = 0; a < D; ++a)
 #pragma unroll
-            for (int b = 0; b < D; ++b) {
-                float s = 0.0f;
+                for (int b = 0; b < D; ++b) {
+                    float s = 0.0f;
 #pragma unroll
-                for (int q = 0; q < D; ++q) {
-                    const float id1 = (q == a && (a < 2 || sp.identity_mode == 1)) ? 1.0f : 0.0f;
-                    s += (id1 + sp.dt * Cn.a[q][a]) * ldg(gout, gstride, L::F + q * D + b, j);
+                    for (int q = 0; q < D; ++q) {
+                        const float id1 = (q == a && (a < 2 || sp.identity_mode == 1)) ? 1.0f : 0.0f;
+                        s += (id1 + sp.dt * Cn.a[q][a]) * ldg(gout, gstride, L::F + q * D + b, j);
+                    }
+                    gin[int64_t(L::F + a * D + b) * gstride + i] = s;
                 }
-                gin[int64_t(L::F + a * D + b) * gstride + i] = s;
-            }
+            gin[int64_t(L::JP) * gstride + i] = ldg(gout, gstride, L::JP, j);
+        }
     }
     __syncthreads();
     float4* dst = slots_b + int64_t(item) * REG;
@@ -450,7 +641,7 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------------
-// P2G adjoint (src/nclr.h:104-165, 313-337, jelly) per sorted position j
+// P2G adjoint (src/nclr.h:104-165, 313-372) per sorted position j
 // ---------------------------------------------------------------------------------------------
 template <int D, bool EXACT>
 __global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ pid,
@@ -493,7 +684,7 @@ __global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ s
     float dw[D][3];
 #pragma unroll
     for (int a = 0; a < D; ++a) bspline_dweights(fx[a], dw[a]);
-    const M<D> A = fused_affine<D, EXACT>(MPM_JELLY, F, C, Jp, mass, vol, sp);
+    const M<D> A = fused_affine<D, EXACT>(sp.material, F, C, Jp, mass, vol, sp);
     M<D> Abar;
     float fxbar[D], vin[D];
 #pragma unroll
@@ -549,7 +740,7 @@ __global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ s
     }
     // A = -(dt V0) Dinv PF + m C  ->  G = dL/dPF
     const float Dinv = 4.0f * sp.inv_dx * sp.inv_dx;
-    const float e = hardening(MPM_JELLY, Jp);
+    const float e = hardening(sp.material, Jp);
     const float mu = sp.mu0 * e, lambda = sp.lambda0 * e;
     M<D> G;
     float sG = 0.0f;
@@ -596,6 +787,8 @@ __global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ s
             gin[int64_t(L::C + a * D + b) * gstride + i] = mass * Abar.a[a][b];
         }
     const float lambar = sG * (J - 1.0f) * J;
+    // snow: mu = mu0 e, lambda = lambda0 e with e = exp(10 (1 - Jp)), de/dJp = -10 e
+    if (sp.material == MPM_SNOW) gin[int64_t(L::JP) * gstride + i] += (mubar * sp.mu0 + lambar * sp.lambda0) * (-10.0f * e);
     epart[j] = e * (mubar * sp.dmu_dE + lambar * sp.dlambda_dE);
 }
 
@@ -652,21 +845,29 @@ using namespace nclr_dev;
 
 static inline unsigned blocks_of(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
 
-template <int D, int PPT>
+template <int D, int PPT, bool GENERAL, bool EXACT>
 static hipError_t g2p_adjoint_one(const LaunchArgs& a, const AdjointArgs& b) {
     const int64_t items = (a.p.n + 256 * PPT - 1) / (256 * PPT) + a.p.ntiles;
-    k_g2p_adjoint<D, PPT><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.tile_start, a.tile_end,
-                                                                 a.item_offset, a.p, a.gv, b.grad_out, b.grad_in,
-                                                                 b.gstride, b.slots_b);
+    k_g2p_adjoint<D, PPT, GENERAL, EXACT><<<unsigned(items), 256, 0, a.stream>>>(
+        a.state_in, a.perm, a.tile_start, a.tile_end, a.item_offset, a.p, a.gv, b.grad_out, b.grad_in, b.gstride,
+        b.slots_b);
     return hipGetLastError();
+}
+
+template <int D, int PPT>
+static hipError_t g2p_adjoint_material(const LaunchArgs& a, const AdjointArgs& b) {
+    if (a.material == MPM_JELLY) return g2p_adjoint_one<D, PPT, false, true>(a, b);
+    // 2D SVDs have one method; in 3D the forward's choice
+    if (D == 2 || a.exact_svd) return g2p_adjoint_one<D, PPT, true, true>(a, b);
+    return g2p_adjoint_one<D, PPT, true, false>(a, b);
 }
 
 template <int D>
 static hipError_t g2p_adjoint_dispatch(const LaunchArgs& a, const AdjointArgs& b) {
     switch (a.p.chunk) {
-        case 256: return g2p_adjoint_one<D, 1>(a, b);
-        case 1024: return g2p_adjoint_one<D, 4>(a, b);
-        default: return g2p_adjoint_one<D, 2>(a, b);
+        case 256: return g2p_adjoint_material<D, 1>(a, b);
+        case 1024: return g2p_adjoint_material<D, 4>(a, b);
+        default: return g2p_adjoint_material<D, 2>(a, b);
     }
 }
 

@@ -410,7 +410,7 @@ struct nclr_mpm_sim {
     }
 
     // ---- state snapshots and the adjoint -------------------------------------------------------
-    int grad_channels() const { return 2 * dim + 2 * dim * dim; }
+    int grad_channels() const { return 2 * dim + 2 * dim * dim + 1; }
     size_t state_floats() const { return size_t(nf()) * size_t(p.stride); }
     size_t state_bytes() const { return state_floats() * sizeof(float) + size_t(p.stride) * sizeof(uint32_t); }
 
@@ -443,17 +443,10 @@ struct nclr_mpm_sim {
         return MPM_OK;
     }
 
-    int check_differentiable() const {
-        for (const auto& sp : scenes)
-            if (sp.material != MPM_JELLY)
-                return fail(MPM_ERR_UNSUPPORTED, "the adjoint step is implemented for jelly scenes only");
-        return MPM_OK;
-    }
 
     // One adjoint step of the current state: recompute sort, P2G and the grid update, then run the
     // adjoint kernels.  The state and the step counter are left unchanged.
     int backward_step(const float* gout, float* gin, double* grad_E) {
-        RC_TRY(check_differentiable());
         RC_TRY(ensure_adjoint());
         if (p.n == 0) return MPM_OK;
         const nclr_host::LaunchArgs a = args(cur);
@@ -473,7 +466,6 @@ struct nclr_mpm_sim {
     // Adjoint of `nsteps` steps from the current state (a checkpoint): the states inside the
     // segment are recomputed into the tape, then the adjoint steps run in reverse.
     int backward_segment(int nsteps, const float* gout, float* gin, double* grad_E) {
-        RC_TRY(check_differentiable());
         RC_TRY(ensure_adjoint());
         if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
         if (p.n == 0) return MPM_OK;
@@ -952,7 +944,7 @@ int mpm_svd(int dim, int64_t n, const float* A, float* U, float* sigma, float* V
     return MPM_OK;
 }
 
-int mpm_grad_channels(int dim) { return (dim == 2 || dim == 3) ? 2 * dim + 2 * dim * dim : -1; }
+int mpm_grad_channels(int dim) { return (dim == 2 || dim == 3) ? 2 * dim + 2 * dim * dim + 1 : -1; }
 
 int64_t mpm_state_bytes(mpm_handle h) { return h ? int64_t(h->state_bytes()) : -1; }
 

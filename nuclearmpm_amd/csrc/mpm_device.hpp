@@ -11,7 +11,8 @@
 namespace nclr_dev {
 
 // Particle fields: x[D] v[D] F[DxD] C[DxD] Jp mass volume, field f of slot i at st[f*stride + i].
-// The first NG = 2D + 2D^2 fields are also the channels of a particle-state cotangent.
+// The first NG = 2D + 2D^2 + 1 fields (x, v, F, C, Jp) are the channels of a state image and of a
+// particle-state cotangent; mass and volume are constants of the particle.
 template <int D>
 struct Layout {
     static constexpr int X = 0;
@@ -22,7 +23,7 @@ struct Layout {
     static constexpr int MASS = JP + 1;
     static constexpr int VOL = JP + 2;
     static constexpr int NF = JP + 3;
-    static constexpr int NG = JP;
+    static constexpr int NG = JP + 1;
 };
 
 // Largest scene index s with key(s) <= v (scenes ordered by every base).

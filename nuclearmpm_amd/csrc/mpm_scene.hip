@@ -2,10 +2,10 @@
 // src/nclr_math.h (cube 100-129, nc_rand_int / nc_rand / randvec 22-39, gyroid 131-139, sample
 // 141-147, pt_3d_to_2d 149-167), producing particle state directly in HBM.
 //
-// Output format ("state image"): float channels x (D), v (D), F (D*D, row-major), C (D*D), each a
-// row of n floats, particle i at column i; mpm_set_state_device takes exactly this image.  Every
+// Output format ("state image"): float channels x (D), v (D), F (D*D, row-major), C (D*D), Jp, each
+// a row of n floats, particle i at column i; mpm_set_state_device takes exactly this image.  Every
 // generator writes all channels: v from the shape, F = diag<D>(1) (diag(1, 1, 0) in 3D for the
-// reference identity mode, I for the physical one), C = 0.
+// reference identity mode, I for the physical one), C = 0, Jp = 1.
 //
 // Random draws reproduce the reference's global xorshift128 stream (Marsaglia 2003, the
 // transition of src/nclr_math.h:22-29; uniform = u32 * 2^-32 in float, src/nclr_math.h:31) in
@@ -34,7 +34,7 @@ __device__ __forceinline__ float linspaced_at(int i, int n, float lo, float hi) 
     return i == size1 ? hi : lo + float(i) * step;
 }
 
-// v, F = diag<D>(1) (or I), C = 0 of particle i; x is written by the caller.
+// v, F = diag<D>(1) (or I), C = 0, Jp = 1 of particle i; x is written by the caller.
 template <int D>
 __device__ __forceinline__ void store_defaults(float* __restrict__ img, int64_t n, int64_t i, const float* v,
                                                int identity_mode) {
@@ -48,6 +48,7 @@ __device__ __forceinline__ void store_defaults(float* __restrict__ img, int64_t 
             img[int64_t(2 * D + a * D + b) * n + i] = f;
             img[int64_t(2 * D + D * D + a * D + b) * n + i] = 0.0f;
         }
+    img[int64_t(2 * D + 2 * D * D) * n + i] = 1.0f;
 }
 
 // Lattice coordinate of point l of a cube, last axis fastest (src/nclr_math.h:107-122).

@@ -1,10 +1,10 @@
 """Scene generators on the device: the reference's scene helpers (src/nclr_math.h:22-39, 100-147)
 as HIP kernels that write particle state straight into HBM (torch tensors on a ROCm device).
 
-A generator returns a *state image*: a float32 tensor of shape (2d + 2d^2, n), rows x[d], v[d],
-F[d*d], C[d*d] (matrices row-major), column i = particle i, with F = diag<d>(1) (diag(1, 1, 0) in
-3D for identity_mode 0, as the reference's Particle ctor builds it; I for identity_mode 1) and
-C = 0.  ``MPMSimulation.from_image`` / ``MPMBatch.from_image`` start an engine from one.
+A generator returns a *state image*: a float32 tensor of shape (2d + 2d^2 + 1, n), rows x[d], v[d],
+F[d*d], C[d*d] (matrices row-major), Jp, column i = particle i, with F = diag<d>(1) (diag(1, 1, 0)
+in 3D for identity_mode 0, as the reference's Particle ctor builds it; I for identity_mode 1),
+C = 0 and Jp = 1.  ``MPMSimulation.from_image`` / ``MPMBatch.from_image`` start an engine from one.
 
     img = generators.cubes(3, scenes.four_cubes_3d().extra["cubes"])   # the benchmark scene
     sim = MPMSimulation.from_image(img, MaterialModel.kSnow, res=128)
@@ -33,8 +33,8 @@ def _torch():
 
 
 def channels(dim: int) -> int:
-    """Rows of a state image: x, v (dim each), F, C (dim^2 each)."""
-    return 2 * dim + 2 * dim * dim
+    """Rows of a state image: x, v (dim each), F, C (dim^2 each), Jp."""
+    return 2 * dim + 2 * dim * dim + 1
 
 
 def _device_index(device) -> int:

@@ -129,9 +129,9 @@ class Particles:
 def _image_dim(image) -> int:
     rows = int(image.shape[0])
     for d in (2, 3):
-        if rows == 2 * d + 2 * d * d:
+        if rows == 2 * d + 2 * d * d + 1:
             return d
-    raise ValueError(f"a state image has 12 (2D) or 24 (3D) rows, not {rows}")
+    raise ValueError(f"a state image has 13 (2D) or 25 (3D) rows, not {rows}")
 
 
 def _placeholder(n: int, dim: int) -> Particles:
@@ -165,12 +165,12 @@ class _Engine:
         self._cache = None
 
     def set_state_image(self, image) -> None:
-        """Sets x, v, F, C of every particle from a device state image: a float32 tensor of shape
-        (2d + 2d^2, n) on the engine's device, rows x, v, F, C (row-major), particle order (the
-        format of nuclearmpm_amd.generators).  Jp, mass and volume are kept."""
+        """Sets x, v, F, C, Jp of every particle from a device state image: a float32 tensor of shape
+        (2d + 2d^2 + 1, n) on the engine's device, rows x, v, F, C (row-major), Jp, particle order
+        (the format of nuclearmpm_amd.generators).  Mass and volume are kept."""
         import torch
 
-        rows = 2 * self._dim + 2 * self._dim * self._dim
+        rows = 2 * self._dim + 2 * self._dim * self._dim + 1
         if (not isinstance(image, torch.Tensor) or image.dtype != torch.float32 or image.device.type != "cuda"
                 or tuple(image.shape) != (rows, self._n)):
             raise ValueError(f"state image must be a float32 cuda tensor of shape ({rows}, {self._n})")
@@ -181,10 +181,10 @@ class _Engine:
         check(lib().mpm_synchronize(self._handle()))
 
     def state_image(self):
-        """The current x, v, F, C as a device state image (particle order)."""
+        """The current x, v, F, C, Jp as a device state image (particle order)."""
         import torch
 
-        rows = 2 * self._dim + 2 * self._dim * self._dim
+        rows = 2 * self._dim + 2 * self._dim * self._dim + 1
         out = torch.empty((rows, self._n), dtype=torch.float32, device=f"cuda:{self.options.device}")
         check(lib().mpm_synchronize(self._handle()))
         check(lib().mpm_get_state_device(self._handle(), out.data_ptr() if self._n else None))

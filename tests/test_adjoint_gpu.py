@@ -60,7 +60,8 @@ def test_one_step_vjp_matches_fp64_oracle_adjoint(d, side, res):
         E = torch.tensor([1e4], dtype=torch.float32)
         E.requires_grad_(True)
         out = autodiff.rollout(sim, s0, E, steps=1, checkpoint_every=1)
-        go = autodiff.pack(*(torch.from_numpy(cot[k]).float().cuda() for k in FIELDS))
+        go = autodiff.pack(*(torch.from_numpy(cot[k]).float().cuda() for k in FIELDS),
+                           Jp=torch.zeros(n, device="cuda"))
         (out * go).sum().backward()
         g = autodiff.unpack(s0.grad.detach(), d)
         gE = float(E.grad)
@@ -102,10 +103,10 @@ def test_backward_step_c_abi_and_permutations():
         np.testing.assert_array_equal(sim.particles().x, mid.x)
     gun = gu.cpu().numpy().astype(np.float64)
     G = O.Grads(*(np.ascontiguousarray(gun[s].T).reshape(getattr(mid, k).shape) for k, s in
-                  zip(FIELDS, (slice(0, 2), slice(2, 4), slice(4, 8), slice(8, 12)))))
+                  zip(FIELDS, (slice(0, 2), slice(2, 4), slice(4, 8), slice(8, 12)))), Jp=gun[12])
     ref = O.adjoint_jelly(to_oracle(mid), G, res=64)
     got = g0.cpu().numpy()
-    for k, s in zip(FIELDS, (slice(0, 2), slice(2, 4), slice(4, 8), slice(8, 12))):
+    for k, s in zip(FIELDS + ("Jp",), (slice(0, 2), slice(2, 4), slice(4, 8), slice(8, 12), slice(12, 13))):
         e = rel_err(got[s].T.reshape(getattr(ref, k).shape), getattr(ref, k))
         assert e < 1e-4, (k, e)
     assert abs(float(gE) - ref.E) <= 1e-4 * abs(ref.E)
@@ -149,7 +150,8 @@ def test_multi_step_gradient_matches_fp64_adjoint_and_finite_differences(d):
         s0 = to_state(p).requires_grad_(True)
         E = torch.tensor([E0], dtype=torch.float64, requires_grad=True)
         out = autodiff.rollout(sim, s0, E, steps, every)
-        w = autodiff.pack(*(torch.from_numpy(getattr(G, k)).float().cuda() for k in FIELDS))
+        w = autodiff.pack(*(torch.from_numpy(getattr(G, k)).float().cuda() for k in FIELDS),
+                          Jp=torch.zeros(n, device="cuda"))
         (out * w).sum().backward()
         g = [t.detach().cpu().numpy().astype(np.float64) for t in autodiff.unpack(s0.grad, d)]
         gE = float(E.grad)
@@ -245,11 +247,100 @@ def test_batched_gradients_equal_single_scene_gradients():
         start += nps[s]
 
 
-def test_non_jelly_scenes_are_rejected():
-    p = Particles.from_positions(scenes.cube(2, 10, 0.4, 0.6))
-    with MPMSimulation(p, MaterialModel.kSnow, res=64) as sim:
+# ---- snow and liquid -----------------------------------------------------------------------
+
+def plastic_particles(d, side, seed):
+    """Particles with F = R diag(s) Q, s drawn on both sides of the snow clamp [0.975, 1.0045] but
+    away from it, Jp in [0.9, 1.1], random v and C (physical identity mode)."""
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0.4, 0.6, side)
+    x = np.stack(np.meshgrid(*([g] * d), indexing="ij"), -1).reshape(-1, d)
+    x = (x + rng.uniform(-0.01, 0.01, x.shape)).astype(np.float32)
+    n = x.shape[0]
+    p = Particles.from_positions(x, identity_mode=1)
+    p.v = rng.normal(0, 0.5, (n, d)).astype(np.float32)
+    F = np.empty((n, d, d))
+    for i in range(n):
+        sig = rng.choice([0.96, 0.985, 0.995, 1.0, 1.002, 1.02], size=d) + rng.uniform(-1e-3, 1e-3, d)
+        Rm, _ = np.linalg.qr(rng.normal(size=(d, d)))
+        Qm, _ = np.linalg.qr(rng.normal(size=(d, d)))
+        Rm[:, 0] *= np.sign(np.linalg.det(Rm))
+        Qm[:, 0] *= np.sign(np.linalg.det(Qm))
+        F[i] = Rm @ np.diag(sig) @ Qm
+    p.F = F.astype(np.float32)
+    p.C = rng.normal(0, 0.5, (n, d, d)).astype(np.float32)
+    p.Jp = rng.uniform(0.9, 1.1, n).astype(np.float32)
+    return p
+
+
+MATS = {"snow": (MaterialModel.kSnow, O.SNOW), "liquid": (MaterialModel.kLiquid, O.LIQUID)}
+
+
+@pytest.mark.parametrize("mat", ["snow", "liquid"])
+@pytest.mark.parametrize("d,side,res", [(2, 20, 64), (3, 8, 32)])
+def test_one_step_vjp_snow_liquid_matches_fp64_oracle_adjoint(mat, d, side, res):
+    """The G2P adjoint through snow's SVD clamp and Jp update and liquid's determinant reset, and
+    the P2G adjoint through snow's hardening, against the FP64 oracle adjoint on the same state."""
+    em, om = MATS[mat]
+    p = plastic_particles(d, side, seed=3 + d)
+    n = p.n
+    rng = np.random.default_rng(17)
+    cot = {k: rng.normal(size=getattr(p, k).shape) for k in FIELDS + ("Jp",)}
+    print(f"[adjoint] one-step VJP {mat} d={d} n={n} res={res}", flush=True)
+    with MPMSimulation(p, em, res=res, E=1e4, identity_mode=1) as sim:
+        s0 = autodiff.pack(*(torch.from_numpy(getattr(p, k)).cuda() for k in FIELDS),
+                           Jp=torch.from_numpy(p.Jp).cuda()).requires_grad_(True)
+        E = torch.tensor([1e4], dtype=torch.float64, requires_grad=True)
+        out = autodiff.rollout(sim, s0, E, steps=1, checkpoint_every=1)
+        go = autodiff.pack(*(torch.from_numpy(cot[k]).float().cuda() for k in FIELDS),
+                           Jp=torch.from_numpy(cot["Jp"]).float().cuda())
+        (out * go).sum().backward()
+        g = autodiff.unpack(s0.grad.detach(), d)
+        gE = float(E.grad)
+    G = O.Grads(*(cot[k].astype(np.float32).astype(np.float64) for k in FIELDS),
+                Jp=cot["Jp"].astype(np.float32).astype(np.float64))
+    ref = O.adjoint(to_oracle(p), G, om, res=res, E=1e4, identity_mode=1)
+    for k, gk in zip(FIELDS + ("Jp",), g):
+        e = rel_err(gk.cpu().numpy().reshape(getattr(ref, k).shape), getattr(ref, k))
+        print(f"[adjoint] {mat} d={d} d{k}: normwise rel err vs FP64 adjoint {e:.2e}", flush=True)
+        assert e < 1e-4, (k, e)
+    print(f"[adjoint] {mat} d={d} dE gpu={gE:.6e} fp64={ref.E:.6e}", flush=True)
+    assert abs(gE - ref.E) <= 1e-4 * abs(ref.E) + 1e-6 * ref.E_abs, (gE, ref.E)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_multi_step_snow_gradient_matches_fp64_adjoint(d):
+    """A snow rollout's gradients (checkpointed) against the FP64 oracle's chained adjoint: a
+    deforming cube whose F leaves the elastic range, so the clamp, Jp and the hardening all act."""
+    side, res, steps, every = (16, 64, 24, 8) if d == 2 else (6, 32, 12, 4)
+    p = affine_jelly(d, side, 0.35, 0.55, seed=20 + d)
+    p.v *= 4.0
+    # the physical identity: the reference's 3D F = diag(1, 1, 0) makes every particle singular,
+    # where the signed SVD's clamp is ill-conditioned (its FP32 and FP64 forwards already disagree)
+    p.F = np.broadcast_to(np.eye(d, dtype=np.float32), p.F.shape).copy()
+    n = p.n
+    rng = np.random.default_rng(27)
+    G = O.Grads(x=rng.normal(size=(n, d)), v=rng.normal(size=(n, d)), F=rng.normal(size=(n, d, d)),
+                C=np.zeros((n, d, d)), Jp=rng.normal(size=n))
+    print(f"[adjoint] multi-step snow d={d} n={n} steps={steps} k={every}", flush=True)
+    with MPMSimulation(p, MaterialModel.kSnow, res=res, E=1e4, identity_mode=1) as sim:
         s0 = to_state(p).requires_grad_(True)
-        E = torch.tensor([1e4], requires_grad=True)
-        out = autodiff.rollout(sim, s0, E, 2, 2)
-        with pytest.raises(NotImplementedError):
-            out.sum().backward()
+        E = torch.tensor([1e4], dtype=torch.float64, requires_grad=True)
+        out = autodiff.rollout(sim, s0, E, steps, every)
+        w = autodiff.pack(*(torch.from_numpy(getattr(G, k)).float().cuda() for k in FIELDS),
+                          Jp=torch.from_numpy(G.Jp).float().cuda())
+        (out * w).sum().backward()
+        g = [t.detach().cpu().numpy().astype(np.float64) for t in autodiff.unpack(s0.grad, d)]
+        gE = float(E.grad)
+        jp_end = autodiff.unpack(out.detach(), d)[4].cpu().numpy()
+    print(f"[adjoint] snow d={d}: Jp after the rollout in [{jp_end.min():.4f}, {jp_end.max():.4f}]", flush=True)
+    assert jp_end.min() < 0.999 or jp_end.max() > 1.001, "the scene never left the elastic range"
+    G32 = O.Grads(*(getattr(G, k).astype(np.float32).astype(np.float64) for k in FIELDS),
+                  Jp=G.Jp.astype(np.float32).astype(np.float64))
+    ref = O.rollout_vjp(to_oracle(p), G32, steps, O.SNOW, res=res, E=1e4, identity_mode=1)
+    for k, gk in zip(FIELDS + ("Jp",), g):
+        e = rel_err(gk.reshape(getattr(ref, k).shape), getattr(ref, k))
+        print(f"[adjoint] snow d={d} d{k}: normwise rel err vs FP64 chained adjoint {e:.2e}", flush=True)
+        assert e < 1e-3, (k, e)
+    print(f"[adjoint] snow d={d} dE gpu={gE:.6e} fp64={ref.E:.6e}", flush=True)
+    assert abs(gE - ref.E) <= 1e-3 * abs(ref.E) + 1e-6 * ref.E_abs, (gE, ref.E)

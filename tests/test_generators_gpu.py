@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 def host_image(p, dim):
     n = p.x.shape[0]
-    return np.concatenate([p.x.T, p.v.T, p.F.reshape(n, -1).T, p.C.reshape(n, -1).T]).astype(np.float32)
+    return np.concatenate([p.x.T, p.v.T, p.F.reshape(n, -1).T, p.C.reshape(n, -1).T, p.Jp[None]]).astype(np.float32)
 
 
 def test_cubes_rebuild_the_benchmark_scenes_bit_for_bit():
@@ -58,9 +58,9 @@ def test_sample_matches_oracle():
     img = generators.sample(3, centers, r).cpu().numpy()
     o = SO.NcRand()
     want = SO.sample(3, centers, o)
-    assert img.shape == (24, 3000)
+    assert img.shape == (25, 3000)
     assert np.array_equal(img[:3].T, want)
-    assert not img[3:6].any() and not img[15:].any()
+    assert not img[3:6].any() and not img[15:24].any() and (img[24] == 1).all()
     assert r.state == o.state
     img2 = generators.sample(2, [[0.5, 0.5]], r, per_center=10, half_width=0.05).cpu().numpy()
     assert np.array_equal(img2[:2].T, SO.sample(2, np.float32([[0.5, 0.5]]), o, per_center=10, half=0.05))

@@ -90,10 +90,12 @@ def test_engine_vjp_matches_fixture(path):
     d = int(g["dim"])
     p = Particles(*(g[f"in_{k}"] for k in FIELDS))
     with MPMSimulation(p, MaterialModel.kJelly, res=kw["res"], E=kw["E"], identity_mode=kw["identity_mode"]) as sim:
-        s0 = autodiff.pack(*(torch.from_numpy(g[f"in_{k}"]).cuda() for k in ("x", "v", "F", "C"))).requires_grad_(True)
+        s0 = autodiff.pack(*(torch.from_numpy(g[f"in_{k}"]).cuda() for k in ("x", "v", "F", "C", "Jp")))
+        s0.requires_grad_(True)
         E = torch.tensor([kw["E"]], dtype=torch.float64, requires_grad=True)
         out = autodiff.rollout(sim, s0, E, steps=1, checkpoint_every=1)
-        cot = autodiff.pack(*(torch.from_numpy(g[f"cot_{k}"]).cuda() for k in ("x", "v", "F", "C")))
+        cot = autodiff.pack(*(torch.from_numpy(g[f"cot_{k}"]).cuda() for k in ("x", "v", "F", "C")),
+                            Jp=torch.zeros(p.n, device="cuda"))
         (out * cot).sum().backward()
         got = [t.detach().cpu().numpy().astype(np.float64) for t in autodiff.unpack(s0.grad, d)]
         gE = float(E.grad)

@@ -60,19 +60,22 @@ def test_state_channels_and_packing():
     from nuclearmpm_amd import autodiff
 
     lib = nm.lib()
-    assert [lib.mpm_grad_channels(d) for d in (1, 2, 3, 4)] == [-1, 12, 24, -1]
-    assert autodiff.channels(2) == 12 and autodiff.channels(3) == 24
+    assert [lib.mpm_grad_channels(d) for d in (1, 2, 3, 4)] == [-1, 13, 25, -1]
+    assert autodiff.channels(2) == 13 and autodiff.channels(3) == 25
     n, d = 5, 3
     g = torch.Generator().manual_seed(0)
     x, v = torch.randn(n, d, generator=g), torch.randn(n, d, generator=g)
     F, C = torch.randn(n, d, d, generator=g), torch.randn(n, d, d, generator=g)
-    s = autodiff.pack(x, v, F, C)
-    assert s.shape == (24, n) and s.is_contiguous()
-    # channel layout: x, v, F row-major, C row-major
+    Jp = torch.rand(n, generator=g)
+    s = autodiff.pack(x, v, F, C, Jp)
+    assert s.shape == (25, n) and s.is_contiguous()
+    # channel layout: x, v, F row-major, C row-major, Jp
     assert torch.equal(s[0:3].t(), x) and torch.equal(s[3:6].t(), v)
     assert torch.equal(s[6 + 1, 2], F[2, 0, 1]) and torch.equal(s[15 + 5, 4], C[4, 1, 2])
-    for a, b in zip(autodiff.unpack(s, d), (x, v, F, C)):
+    assert torch.equal(s[24], Jp)
+    for a, b in zip(autodiff.unpack(s, d), (x, v, F, C, Jp)):
         assert torch.equal(a, b)
+    assert torch.equal(autodiff.pack(x, v, F, C)[24], torch.ones(n))  # the Particle ctor's Jp
 
 
 def test_create_rejects_bad_arguments_before_touching_the_device():

@@ -53,7 +53,7 @@ def slot_order(sim):
     from nuclearmpm_amd._lib import check, lib
 
     d, n = sim.dim, sim.n
-    src = torch.arange(n, dtype=torch.float32, device="cuda").repeat(2 * d + 2 * d * d, 1)
+    src = torch.arange(n, dtype=torch.float32, device="cuda").repeat(lib().mpm_grad_channels(d), 1)
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     check(lib().mpm_grad_from_particle_order(sim._handle(), src.data_ptr(), dst.data_ptr()))
