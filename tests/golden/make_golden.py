@@ -2,8 +2,8 @@
 
 Each fixture is a small seeded scene advanced a few steps by the FP32 oracle to a
 mid-trajectory state (so that C, F and Jp are non-trivial), followed by one more step in FP32 and
-in FP64 from that state.  Jelly fixtures also carry a random cotangent of the state after the
-step and the FP64 vector-Jacobian product of the step.
+in FP64 from that state.  Every fixture also carries a random cotangent of the state after the
+step (x, v, F, C, Jp) and the FP64 vector-Jacobian product of the step.
 
     python tests/golden/make_golden.py        # rewrites tests/golden/*.npz
 
@@ -71,18 +71,18 @@ def make(name, d, material, res, idm, warm, E, seed):
         rec[f"in_{k}"] = getattr(mid, k).astype(np.float32)
         rec[f"out32_{k}"] = getattr(out32, k).astype(np.float32)
         rec[f"out64_{k}"] = getattr(out64, k).astype(np.float64)
-    if material == "jelly":
-        rng = np.random.default_rng(seed + 100)
-        n = mid.n
-        g = O.Grads(x=rng.normal(size=(n, d)), v=rng.normal(size=(n, d)), F=rng.normal(size=(n, d, d)),
-                    C=rng.normal(size=(n, d, d)))
-        g = O.Grads(*(getattr(g, k).astype(np.float32).astype(np.float64) for k in ("x", "v", "F", "C")))
-        vjp = O.adjoint_jelly(mid, g, **kw)
-        for k in ("x", "v", "F", "C"):
-            rec[f"cot_{k}"] = getattr(g, k).astype(np.float32)
-            rec[f"vjp_{k}"] = getattr(vjp, k)
-        rec["vjp_E"] = np.float64(vjp.E)
-        rec["vjp_E_abs"] = np.float64(vjp.E_abs)
+    rng = np.random.default_rng(seed + 100)
+    n = mid.n
+    g = O.Grads(x=rng.normal(size=(n, d)), v=rng.normal(size=(n, d)), F=rng.normal(size=(n, d, d)),
+                C=rng.normal(size=(n, d, d)), Jp=rng.normal(size=n))
+    g = O.Grads(*(getattr(g, k).astype(np.float32).astype(np.float64) for k in ("x", "v", "F", "C")),
+                Jp=g.Jp.astype(np.float32).astype(np.float64))
+    vjp = O.adjoint(mid, g, mat, **kw)
+    for k in ("x", "v", "F", "C", "Jp"):
+        rec[f"cot_{k}"] = getattr(g, k).astype(np.float32)
+        rec[f"vjp_{k}"] = getattr(vjp, k)
+    rec["vjp_E"] = np.float64(vjp.E)
+    rec["vjp_E_abs"] = np.float64(vjp.E_abs)
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **{k: np.asarray(v) for k, v in rec.items()})
     return mid.n
 

@@ -1,7 +1,7 @@
 """Golden step fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py from the
 oracle): small seeded two-cube scenes for every dimension and material, at a mid-trajectory
-state, with the one-step result in FP32 and FP64 and, for jelly, a cotangent and its FP64
-vector-Jacobian product.
+state, with the one-step result in FP32 and FP64, a cotangent and its FP64 vector-Jacobian
+product.
 
 CPU: the oracle still reproduces every fixture bit for bit.
 GPU: the engine's step from each fixture's state meets the parity bound against the FP32 result,
@@ -49,12 +49,12 @@ def test_oracle_reproduces_fixture(path):
     for k in FIELDS:
         assert np.array_equal(getattr(out32, k), g[f"out32_{k}"]), k
         assert np.array_equal(getattr(out64, k), g[f"out64_{k}"]), k
-    if "vjp_x" in g:
-        cot = O.Grads(*(g[f"cot_{k}"].astype(np.float64) for k in ("x", "v", "F", "C")))
-        vjp = O.adjoint_jelly(mid, cot, **kwargs(g))
-        for k in ("x", "v", "F", "C"):
-            np.testing.assert_array_equal(getattr(vjp, k), g[f"vjp_{k}"])
-        assert vjp.E == float(g["vjp_E"]) and vjp.E_abs == float(g["vjp_E_abs"])
+    cot = O.Grads(*(g[f"cot_{k}"].astype(np.float64) for k in ("x", "v", "F", "C")),
+                  Jp=g["cot_Jp"].astype(np.float64))
+    vjp = O.adjoint(mid, cot, int(g["material"]), **kwargs(g))
+    for k in ("x", "v", "F", "C", "Jp"):
+        np.testing.assert_array_equal(getattr(vjp, k), g[f"vjp_{k}"])
+    assert vjp.E == float(g["vjp_E"]) and vjp.E_abs == float(g["vjp_E_abs"])
 
 
 @pytest.mark.gpu
@@ -79,7 +79,7 @@ def test_engine_step_matches_fixture(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", [f for f in FILES if "jelly" in f], ids=lambda p: os.path.basename(p)[:-4])
+@pytest.mark.parametrize("path", FILES, ids=lambda p: os.path.basename(p)[:-4])
 def test_engine_vjp_matches_fixture(path):
     import torch
 
@@ -89,20 +89,27 @@ def test_engine_vjp_matches_fixture(path):
     kw = kwargs(g)
     d = int(g["dim"])
     p = Particles(*(g[f"in_{k}"] for k in FIELDS))
-    with MPMSimulation(p, MaterialModel.kJelly, res=kw["res"], E=kw["E"], identity_mode=kw["identity_mode"]) as sim:
+    with MPMSimulation(p, MaterialModel(int(g["material"])), res=kw["res"], E=kw["E"],
+                       identity_mode=kw["identity_mode"]) as sim:
         s0 = autodiff.pack(*(torch.from_numpy(g[f"in_{k}"]).cuda() for k in ("x", "v", "F", "C", "Jp")))
         s0.requires_grad_(True)
         E = torch.tensor([kw["E"]], dtype=torch.float64, requires_grad=True)
         out = autodiff.rollout(sim, s0, E, steps=1, checkpoint_every=1)
-        cot = autodiff.pack(*(torch.from_numpy(g[f"cot_{k}"]).cuda() for k in ("x", "v", "F", "C")),
-                            Jp=torch.zeros(p.n, device="cuda"))
+        cot = autodiff.pack(*(torch.from_numpy(g[f"cot_{k}"]).cuda() for k in ("x", "v", "F", "C", "Jp")))
         (out * cot).sum().backward()
         got = [t.detach().cpu().numpy().astype(np.float64) for t in autodiff.unpack(s0.grad, d)]
         gE = float(E.grad)
-    for k, gk in zip(("x", "v", "F", "C"), got):
+    # x's cotangent sums, over every stencil node, weight-derivative terms of both signs through the
+    # stress: bound 2e-4 (measured at most 1.05e-4 here).  Snow's Jp cotangent goes through the
+    # hardening, d(mu, lambda)/dJp, i.e. through the stress's sensitivity to mu, 2 (F - R) F^T, whose
+    # FP32 rounding is eps / strain relative at small strain, the conditioning of dL/dE below:
+    # bound 1e-3 (measured 5.5e-4 on the 3D snow state).  The other fields: 1e-4.
+    bounds = {"x": 2e-4, "Jp": 1e-3}
+    for k, gk in zip(("x", "v", "F", "C", "Jp"), got):
         ref = g[f"vjp_{k}"]
-        e = float(np.abs(gk.reshape(ref.shape) - ref).max() / np.abs(ref).max())
-        assert e < 1e-4, (k, e)
+        e = float(np.abs(gk.reshape(ref.shape) - ref).max() / max(np.abs(ref).max(), 1e-30))
+        print(f"[golden] {os.path.basename(path)[:-4]}: VJP d{k} normwise rel err {e:.2e}", flush=True)
+        assert e < bounds.get(k, 1e-4), (k, e)
     # dL/dE sums per-particle terms that cancel (here 50x), and each term carries the FP32 rounding
     # of F - R, relative eps / strain at small strain (the forward stress's own conditioning): the
     # bound is 1e-3 of the sum of the terms' magnitudes
