@@ -16,8 +16,10 @@
 //               *stamp marks the node tiles the current step touched.
 //
 // Kernels per step
-//   keys   tile key per particle; bumps the device step stamp; clears the tile bounds
-//   sort   stable radix counting sort of (key, slot); tile bounds; P2G work items
+//   sort   stable counting sort of the slots by tile: after the first step the incremental bucket
+//          sort of mpm_bucket.hip (a particle's tile is a neighbour of its previous one); at
+//          creation and after the state is replaced, keys (tile key per particle, the device step
+//          stamp), the radix sort of mpm_sort.hip, tile bounds and P2G work items
 //   p2g    one workgroup per work item: fused affine matrix per particle, quantised 3^D
 //          stencil contributions summed exactly with 64-bit LDS integer atomics, region -> slot
 //   grid   one workgroup per node tile: sums the slots covering each node, grid_op
@@ -402,8 +404,30 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
                 r = r * TL::R + ra;
             }
             if (!inside) continue;
-            for (int it = 0; it < count[c]; ++it) {
-                const float4 s = slots[int64_t(first[c] + it) * TL::REGION + r];
+            // items in order; four loads in flight before their (ordered) sums
+            const float4* q = slots + int64_t(first[c]) * TL::REGION + r;
+            int it = 0;
+            for (; it + 4 <= count[c]; it += 4, q += 4 * TL::REGION) {
+                const float4 s0 = q[0], s1 = q[TL::REGION], s2 = q[2 * TL::REGION], s3 = q[3 * TL::REGION];
+                g.x += s0.x;
+                g.y += s0.y;
+                g.z += s0.z;
+                g.w += s0.w;
+                g.x += s1.x;
+                g.y += s1.y;
+                g.z += s1.z;
+                g.w += s1.w;
+                g.x += s2.x;
+                g.y += s2.y;
+                g.z += s2.z;
+                g.w += s2.w;
+                g.x += s3.x;
+                g.y += s3.y;
+                g.z += s3.z;
+                g.w += s3.w;
+            }
+            for (; it < count[c]; ++it, q += TL::REGION) {
+                const float4 s = q[0];
                 g.x += s.x;
                 g.y += s.y;
                 g.z += s.z;
