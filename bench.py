@@ -88,6 +88,25 @@ def cpu_baseline(sc, steps: int):
     }
 
 
+def two_cubes_2d():
+    """The reference example's 2D snow scene (src/example.cpp:29-46: two 50x50 cubes, 5,000
+    particles, res 64) for its 4,000 steps as hipGraph replays: tiny, so bounded by the per-step
+    launch sequence rather than by HBM."""
+    from nuclearmpm_amd import MPMSimulation, scenes
+
+    sc = scenes.two_cubes_2d()
+    with MPMSimulation(sc.particles, sc.material, res=sc.res, graph=True) as sim:
+        sim.advance(20)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.advance(sc.steps)
+        sim.synchronize()
+        t = time.perf_counter() - t0
+    return {"workload": "two_cubes_2d: 2D snow, 2 x 50x50 particles, res 64, dt 1e-4 (src/example.cpp)",
+            "particles": sc.particles.n, "steps": sc.steps, "value": sc.particles.n * sc.steps / t,
+            "unit": "particle-steps/s", "us_per_step": 1e6 * t / sc.steps}
+
+
 def batched_vs_sequential(steps: int):
     """2D batched workload: 256 seeded scenes (1-4 cubes each, liquid / snow / jelly, res 128) in
     one launch sequence per step, against the same scenes advanced one plain simulation at a time
@@ -350,7 +369,8 @@ def main() -> int:
         "state_finite": finite,
     }
     if not args.no_extra:
-        legs = (("batched", lambda: batched_vs_sequential(200)),
+        legs = (("two_cubes_2d", two_cubes_2d),
+                ("batched", lambda: batched_vs_sequential(200)),
                 ("fwd_bwd_2d", lambda: fwd_bwd("jelly_2d_diff")),
                 ("fwd_bwd_3d", lambda: fwd_bwd("jelly_3d_batched")),
                 ("scene_gen", scene_generation))
