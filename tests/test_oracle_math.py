@@ -195,3 +195,15 @@ def test_linspaced_matches_lattice_definition():
     v32 = O.linspaced(50, np.float32(0.4), np.float32(0.6))
     assert v32.dtype == np.float32 and v32[-1] == np.float32(0.6)
     assert np.abs(v32 - np.linspace(0.4, 0.6, 50)).max() < 1e-7
+
+
+def test_polar_2d_of_a_scaled_reflection_is_not_finite():
+    """The reference's 2D polar closed form (src/nclr_math.h:76-98) divides by
+    sqrt((F00 + F11)^2 + (F10 - F01)^2), which is zero for a scaled reflection [[c, s], [s, -c]]:
+    R is NaN.  2D snow reaches such an F when both singular values of a reflected F^ (nclr_svd's
+    2D singular values are non-negative) are clamped to the same bound, e.g. 0.975 U V^T."""
+    c, s = np.float32(0.6), np.float32(0.8)
+    F = (np.float32(0.975) * np.array([[c, s], [s, -c]], np.float32)).astype(np.float32)
+    assert F[0, 0] + F[1, 1] == 0 and F[1, 0] - F[0, 1] == 0
+    R = O.polar(F[None])
+    assert not np.isfinite(R).any()

@@ -170,6 +170,27 @@ def test_out_of_range_raises():
             sim.synchronize()
 
 
+def test_snow_2d_scaled_reflection_gives_the_reference_nan():
+    """A 2D snow particle whose F is a scaled reflection has a non-finite polar factor in the
+    reference's closed form, so the step's stress and the particle's next position are NaN (the
+    oracle's step agrees), and the following step reports it (IndexError, as a NaN position's
+    grid index throws in the reference).
+    The 256-scene batched 2D workload reaches this state in one snow scene after ~478 steps."""
+    c, s = np.float32(0.6), np.float32(0.8)
+    F = (np.float32(0.975) * np.array([[c, s], [s, -c]], np.float32)).astype(np.float32)
+    x = np.array([[0.5, 0.5], [0.52, 0.5]], np.float32)
+    p = Particles.from_positions(x)
+    p.F[0] = F
+    out = O.run(to_oracle(p), O.SNOW, res=64, steps=1)
+    assert not np.isfinite(out.x[0]).all()
+    with MPMSimulation(p, MaterialModel.kSnow, res=64) as sim:
+        sim.advance(1)
+        assert not np.isfinite(sim.particles().x[0]).all()
+        sim.advance(1)
+        with pytest.raises(IndexError):
+            sim.synchronize()
+
+
 def test_kernel_timing_and_params():
     sc = scenes.two_cubes_2d()
     with MPMSimulation(sc.particles, sc.material, res=sc.res, timing=True) as sim:
