@@ -323,6 +323,11 @@ def main() -> int:
         "grid": {"avg_ms": kt["grid"], "alg_bytes": grid_bytes},
         "g2p": {"avg_ms": kt["g2p"], "alg_bytes": g2p_bytes},
     }
+    # P2G accumulates every particle's 3^d x (d + 1) quantised contributions with 64-bit integer
+    # atomics in LDS (no HBM atomics: each work item stores its own slot)
+    lds_atomics = (3 ** dim) * (dim + 1) * n
+    per_kernel["p2g"]["lds_atomic_u64_per_launch"] = lds_atomics
+    per_kernel["p2g"]["lds_atomics_per_s"] = lds_atomics / (kt["p2g"] * 1e-3) if kt["p2g"] > 0 else 0.0
     for v in per_kernel.values():
         v["achieved_GBs"] = v["alg_bytes"] / (v["avg_ms"] * 1e-3) / 1e9 if v["avg_ms"] > 0 else 0.0
         v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
