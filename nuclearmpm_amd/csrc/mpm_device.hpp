@@ -183,10 +183,10 @@ NCLR_DI void fixed_point_scale(float total, float& scale, double& inv_scale) {
     inv_scale = ldexp(1.0, -e);
 }
 
-// round-to-nearest-even integer of y (|y| < 2^62) as a two's complement 64-bit value, in double:
-// r = rint(y) exactly, r = hi 2^32 + lo with lo in [0, 2^32), both halves converted directly
+// floor(y) (|y| < 2^62) as a two's complement 64-bit value, in double: y = hi 2^32 + lo exactly with
+// hi an integer and lo in [0, 2^32); hi converts exactly and lo truncates to its floor
 NCLR_DI unsigned long long fixed_point(float y) {
-    const double r = rint(double(y));
+    const double r = double(y);
     const double hi = floor(r * 2.3283064365386963e-10);  // 2^-32
     const double lo = fma(hi, -4294967296.0, r);           // exact
     return (unsigned long long)(uint32_t)lo | ((unsigned long long)(uint32_t)(int32_t)hi << 32);

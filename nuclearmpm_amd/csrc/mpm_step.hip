@@ -147,6 +147,18 @@ NCLR_DI void p2g_deposit(unsigned long long* __restrict__ acc, uint32_t* __restr
                          const float (&scale)[4]) {
     using TL = Tile<D>;
     constexpr int REG = TL::REGION;
+    // The channel scales are powers of two, and scaling by 2^e commutes with every rounding: the
+    // per-particle factors are scaled once, so each node's weight * (m v_c + A_c dpos) * 2^e comes
+    // out bit for bit without a multiply per node and channel.
+    M<D> As;
+    float mvs[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        mvs[c] = mv[c] * scale[c];
+#pragma unroll
+        for (int b = 0; b < D; ++b) As.a[c][b] = A.a[c][b] * scale[c];
+    }
+    const float ms = m * scale[3];
     float w[D][3], dp[D][3], last[D][3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -157,17 +169,17 @@ NCLR_DI void p2g_deposit(unsigned long long* __restrict__ acc, uint32_t* __restr
 #pragma unroll
     for (int c = 0; c < D; ++c)
 #pragma unroll
-        for (int o = 0; o < 3; ++o) last[c][o] = A.a[c][D - 1] * dp[D - 1][o];
+        for (int o = 0; o < 3; ++o) last[c][o] = As.a[c][D - 1] * dp[D - 1][o];
     auto node = [&](int ro, float weight, const float (&outer)[D], int ol) {
         float val[4];
 #pragma unroll
-        for (int c = 0; c < D; ++c) val[c] = weight * (mv[c] + (outer[c] + last[c][ol]));
-        val[3] = weight * m;
+        for (int c = 0; c < D; ++c) val[c] = weight * (mvs[c] + (outer[c] + last[c][ol]));
+        val[3] = weight * ms;
         uint32_t bad = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             if (D == 2 && c == 2) continue;
-            const float y = val[c] * scale[c];
+            const float y = val[c];
             if (!CHECKED || fabsf(y) < kFixedLimit) atomicAdd(&acc[c * REG + rb + ro], fixed_point(y));
             else bad |= 1u << c;
         }
@@ -180,7 +192,7 @@ NCLR_DI void p2g_deposit(unsigned long long* __restrict__ acc, uint32_t* __restr
             for (int o1 = 0; o1 < 3; ++o1) {
                 float outer[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) outer[c] = A.a[c][0] * dp[0][o0] + A.a[c][1] * dp[1][o1];
+                for (int c = 0; c < 3; ++c) outer[c] = As.a[c][0] * dp[0][o0] + As.a[c][1] * dp[1][o1];
                 const float w01 = w[0][o0] * w[1][o1];
 #pragma unroll
                 for (int o2 = 0; o2 < 3; ++o2) node((o0 * TL::R + o1) * TL::R + o2, w01 * w[2][o2], outer, o2);
@@ -190,7 +202,7 @@ NCLR_DI void p2g_deposit(unsigned long long* __restrict__ acc, uint32_t* __restr
         for (int o0 = 0; o0 < 3; ++o0) {
             float outer[2];
 #pragma unroll
-            for (int c = 0; c < 2; ++c) outer[c] = A.a[c][0] * dp[0][o0];
+            for (int c = 0; c < 2; ++c) outer[c] = As.a[c][0] * dp[0][o0];
 #pragma unroll
             for (int o1 = 0; o1 < 3; ++o1) node(o0 * TL::R + o1, w[0][o0] * w[1][o1], outer, o1);
         }
