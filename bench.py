@@ -90,12 +90,12 @@ def cpu_baseline(sc, steps: int):
 
 def two_cubes_2d():
     """The reference example's 2D snow scene (src/example.cpp:29-46: two 50x50 cubes, 5,000
-    particles, res 64) for its 4,000 steps as hipGraph replays: tiny, so bounded by the per-step
-    launch sequence rather than by HBM."""
+    particles, res 64) for its 4,000 steps: tiny, so bounded by the per-step launch sequence
+    rather than by HBM."""
     from nuclearmpm_amd import MPMSimulation, scenes
 
     sc = scenes.two_cubes_2d()
-    with MPMSimulation(sc.particles, sc.material, res=sc.res, graph=True) as sim:
+    with MPMSimulation(sc.particles, sc.material, res=sc.res) as sim:
         sim.advance(20)
         sim.synchronize()
         t0 = time.perf_counter()
@@ -109,13 +109,13 @@ def two_cubes_2d():
 
 def batched_vs_sequential(steps: int):
     """2D batched workload: 256 seeded scenes (1-4 cubes each, liquid / snow / jelly, res 128) in
-    one launch sequence per step, against the same scenes advanced one plain simulation at a time
-    (both with hipGraph steps); particle-steps/s of each and the ratio."""
+    one launch sequence per step, against the same scenes advanced one plain simulation at a time;
+    particle-steps/s of each and the ratio."""
     from nuclearmpm_amd import MPMBatch, MPMSimulation, scenes
 
     specs = scenes.batched_2d(seed=1, scenes=256)
     n = sum(s.particles.n for s in specs)
-    with MPMBatch(specs, graph=True) as b:
+    with MPMBatch(specs) as b:
         b.advance(5)
         b.synchronize()
         t0 = time.perf_counter()
@@ -125,8 +125,7 @@ def batched_vs_sequential(steps: int):
     seq_steps = max(5, steps // 4)
     ts = 0.0
     for s in specs:
-        with MPMSimulation(s.particles, s.material, res=s.res, dt=s.dt, E=s.E, nu=s.nu, gravity=s.gravity,
-                           graph=True) as sim:
+        with MPMSimulation(s.particles, s.material, res=s.res, dt=s.dt, E=s.E, nu=s.nu, gravity=s.gravity) as sim:
             sim.advance(2)
             sim.synchronize()
             t0 = time.perf_counter()
@@ -252,7 +251,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and forward+backward legs")
-    ap.add_argument("--no-graph", action="store_true", help="launch the step kernels one by one (no hipGraph)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each step as a captured hipGraph (measured slower than direct launches on this "
+                         "ROCm stack for every workload here, so off by default)")
     ap.add_argument("--fast-svd", action="store_true", help="engine option fast_svd (not bitwise the reference)")
     ap.add_argument("--tuning", default="auto",
                     help="P2G particles per work item and G2P workgroup size as 'CHUNKxTHREADS' (e.g. 512x256), "
@@ -283,7 +284,7 @@ def main() -> int:
         del src, dst
 
     sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity,
-                        graph=not args.no_graph, fast_svd=args.fast_svd)
+                        graph=args.graph, fast_svd=args.fast_svd)
     sim.advance(args.warmup)
     sim.synchronize()
     if args.tuning == "auto":
@@ -355,7 +356,7 @@ def main() -> int:
             "dim": dim,
             "material": "snow",
             "mode": "fwd",
-            "hipgraph": not args.no_graph,
+            "hipgraph": args.graph,
             "tuning": {"p2g_chunk": int(tuning["p2g_chunk"]), "g2p_threads": int(tuning["g2p_threads"]),
                        "chosen_by": "autotuner" if args.tuning == "auto" else "flag"},
             "svd": "fast (not bitwise the reference)" if args.fast_svd else "nclr_svd order (bitwise the reference's FP32)",
