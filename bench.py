@@ -262,8 +262,8 @@ def main() -> int:
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the engine has no multi-GPU path in this build "
-              "(scenes shard without communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the engine has no multi-GPU path (batched scenes would "
+              "shard by scene without communication). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     from nuclearmpm_amd import MPMSimulation, scenes
