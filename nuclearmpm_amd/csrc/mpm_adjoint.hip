@@ -328,17 +328,15 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
             for (int a = 0; a < D; ++a)
 #pragma unroll
                 for (int b = 0; b < D; ++b) Cf.a[a][b] = 0.0f;
+            const StencilRows<D> node(gv, base, sp.ntn, sp.node_tile_base);
 #pragma unroll
             for (int o = 0; o < K; ++o) {
-                int off[D], nidx[D];
+                int off[D];
                 float dpos[D];
                 offsets<D>(o, off);
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    dpos[a] = float(off[a]) - fx[a];
-                    nidx[a] = base[a] + off[a];
-                }
-                const float4 g = gv[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
+                for (int a = 0; a < D; ++a) dpos[a] = float(off[a]) - fx[a];
+                const float4 g = node(off);
                 const float gvel[3] = {g.x, g.y, g.z};
                 float weight = w[0][off[0]];
 #pragma unroll
@@ -433,19 +431,19 @@ __global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ s
 #pragma unroll
             for (int b = 0; b < D; ++b) Cn.a[a][b] = 0.0f;
         }
+        const StencilRows<D> node(gv, bases[k], sp.ntn, sp.node_tile_base);
 #pragma unroll
         for (int o = 0; o < K; ++o) {
             int off[D];
             offsets<D>(o, off);
             float d[D];
-            int nidx[D], ro = 0;
+            int ro = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 d[a] = float(off[a]) - fxs[k][a];
-                nidx[a] = bases[k][a] + off[a];
                 ro = ro * TL::R + off[a];
             }
-            const float4 g = gv[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
+            const float4 g = node(off);
             const float gvel[3] = {g.x, g.y, g.z};
             float weight = w[0][off[0]];
 #pragma unroll
@@ -694,18 +692,15 @@ __global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ s
 #pragma unroll
         for (int b = 0; b < D; ++b) Abar.a[a][b] = 0.0f;
     }
+    const StencilRows<D> node(gbar, base, sp.ntn, sp.node_tile_base);
 #pragma unroll
     for (int o = 0; o < K; ++o) {
         int off[D];
         offsets<D>(o, off);
         float dpos[D];
-        int nidx[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            dpos[a] = (float(off[a]) - fx[a]) * sp.dx;
-            nidx[a] = base[a] + off[a];
-        }
-        const float4 g = gbar[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
+        for (int a = 0; a < D; ++a) dpos[a] = (float(off[a]) - fx[a]) * sp.dx;
+        const float4 g = node(off);
         const float gm[3] = {g.x, g.y, g.z};
         float weight = w[0][off[0]];
 #pragma unroll

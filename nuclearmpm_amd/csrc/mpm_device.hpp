@@ -128,6 +128,40 @@ NCLR_DI int64_t gv_index(const int (&n)[D], int ntn, uint32_t node_tile_base) {
     return (int64_t(node_tile_base) + t) * TL::NODES + local;
 }
 
+// The 3^D stencil nodes base + off of one particle in the updated grid, addressed per axis: node
+// n's byte offset inside its scene's node tiles is sum_a t_a ntn^(D-1-a) NODES 16 +
+// l_a S^(D-1-a) 16 with t_a = n_a >> LOG, l_a = n_a & (S-1) (gv_index's layout), so rows[a][o]
+// holds axis a's term for n_a = base[a] + o and a node costs one 32-bit add of D terms.  A scene's
+// node tiles span less than 2^32 bytes (res <= 512 in 3D: 2.25e9; res <= 8192 in 2D: 1.08e9).
+template <int D>
+struct StencilRows {
+    const char* scene;
+    uint32_t rows[D][3];
+
+    NCLR_DI StencilRows(const float4* __restrict__ g, const int (&base)[D], int ntn, uint32_t node_tile_base) {
+        using TL = Tile<D>;
+        scene = reinterpret_cast<const char*>(g + int64_t(node_tile_base) * TL::NODES);
+        uint32_t tstride = TL::NODES * uint32_t(sizeof(float4)), lstride = uint32_t(sizeof(float4));
+#pragma unroll
+        for (int a = D - 1; a >= 0; --a) {
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                const uint32_t n = uint32_t(base[a] + o);
+                rows[a][o] = (n >> TL::LOG) * tstride + (n & uint32_t(TL::S - 1)) * lstride;
+            }
+            tstride *= uint32_t(ntn);
+            lstride *= uint32_t(TL::S);
+        }
+    }
+
+    NCLR_DI float4 operator()(const int (&off)[D]) const {
+        uint32_t b = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) b += rows[a][off[a]];
+        return *reinterpret_cast<const float4*>(scene + b);
+    }
+};
+
 template <int D>
 NCLR_DI uint32_t tile_key(const int (&base)[D], int nt) {
     uint32_t t = 0;

@@ -539,18 +539,15 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
     }
     const float four_inv_dx = 4.0f * sp.inv_dx;
     constexpr int K = D == 2 ? 9 : 27;
+    const StencilRows<D> node(gv, base, sp.ntn, sp.node_tile_base);
 #pragma unroll
     for (int o = 0; o < K; ++o) {
         int off[D];
         offsets<D>(o, off);
         float dpos[D];
-        int nidx[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            dpos[a] = float(off[a]) - fx[a];
-            nidx[a] = base[a] + off[a];
-        }
-        const float4 g = gv[gv_index<D>(nidx, sp.ntn, sp.node_tile_base)];
+        for (int a = 0; a < D; ++a) dpos[a] = float(off[a]) - fx[a];
+        const float4 g = node(off);
         const float gvel[3] = {g.x, g.y, g.z};
         float weight = w[0][off[0]];
 #pragma unroll
