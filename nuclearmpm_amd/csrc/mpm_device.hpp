@@ -217,12 +217,15 @@ NCLR_DI void fixed_point_scale(float total, float& scale, double& inv_scale) {
     inv_scale = ldexp(1.0, -e);
 }
 
-// floor(y) (|y| < 2^62) as a two's complement 64-bit value, in double: y = hi 2^32 + lo exactly with
-// hi an integer and lo in [0, 2^32); hi converts exactly and lo truncates to its floor
+// y (|y| < 2^62) quantised to a two's complement 64-bit value, in double: hi = floor(y 2^-32) is
+// the high word and lo = y - hi 2^32 in [0, 2^32], truncated by a saturating conversion, the low
+// word.  That is floor(y) for every such float except four in (-4, -1] (-(1 + 2^-23), -(1 + 2^-22),
+// -(2 + 2^-22), -(3 + 2^-22)), where 2^32 + y rounds up to an integer and the result is floor(y) + 1:
+// still deterministic and within one unit, which is all the exact sums need.
 NCLR_DI unsigned long long fixed_point(float y) {
     const double r = double(y);
     const double hi = floor(r * 2.3283064365386963e-10);  // 2^-32
-    const double lo = fma(hi, -4294967296.0, r);           // exact
+    const double lo = fma(hi, -4294967296.0, r);
     return (unsigned long long)(uint32_t)lo | ((unsigned long long)(uint32_t)(int32_t)hi << 32);
 }
 
