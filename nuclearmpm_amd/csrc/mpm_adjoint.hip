@@ -28,6 +28,8 @@
 // finite differences of the step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mpm_device.hpp"
 
 namespace nclr_dev {
@@ -787,34 +789,64 @@ __global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ s
     epart[j] = e * (mubar * sp.dmu_dE + lambar * sp.dlambda_dE);
 }
 
-// grad_E[s] += sum of epart over scene s's sorted particles, in a fixed order (one workgroup per
-// scene; scenes occupy consecutive sorted keys).
-__global__ __launch_bounds__(256) void k_scene_e_sum(const uint32_t* __restrict__ keys, const float* __restrict__ epart,
-                                                     StepParams p, double* __restrict__ grad_E) {
-    __shared__ double part[256];
-    __shared__ int64_t s_lo, s_hi;
-    const int sc = blockIdx.x;
-    if (threadIdx.x < 2) {
-        const uint32_t key = threadIdx.x == 0 ? p.scenes[sc].tile_base
-                                              : (sc + 1 < p.nscenes ? p.scenes[sc + 1].tile_base : p.ntiles);
-        int64_t lo = 0, hi = p.n;  // first sorted position with keys[.] >= key
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (keys[mid] < key) lo = mid + 1;
-            else hi = mid;
-        }
-        if (threadIdx.x == 0) s_lo = lo;
-        else s_hi = lo;
+// grad_E[s] += the sum of epart over scene s's sorted particles, in a fixed order, in three
+// kernels: the scene's sorted range [lo, hi) (scenes occupy consecutive keys; particles off their
+// grid sort last), partial sums of consecutive chunks of kEChunk positions counted from lo (each a
+// fixed tree), then the scene's chunks in order.  The partition depends only on the scene's own
+// range, so a scene's dL/dE is the same alone and inside a batch.
+constexpr int kEChunk = 2048;
+
+__global__ __launch_bounds__(64) void k_scene_ranges(const uint32_t* __restrict__ keys, StepParams p,
+                                                     int64_t* __restrict__ range) {
+    const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= 2 * p.nscenes) return;
+    const int sc = t >> 1;
+    const uint32_t key = (t & 1) == 0 ? p.scenes[sc].tile_base
+                                      : (sc + 1 < p.nscenes ? p.scenes[sc + 1].tile_base : p.ntiles);
+    int64_t lo = 0, hi = p.n;  // first sorted position with keys[.] >= key
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
     }
-    __syncthreads();
-    double acc = 0.0;
-    for (int64_t j = s_lo + threadIdx.x; j < s_hi; j += 256) acc += double(epart[j]);
-    part[threadIdx.x] = acc;
-    __syncthreads();
+    range[t] = lo;
+}
+
+// Sum of part[0..256) by a fixed tree; the result is in part[0] for thread 0.
+NCLR_DI void tree_sum_256(double* part) {
     for (int off = 128; off > 0; off >>= 1) {
-        if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
         __syncthreads();
+        if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
     }
+}
+
+// Grid (chunks, scenes): chunk blockIdx.x of scene blockIdx.y -> partial[scene * chunks + chunk].
+__global__ __launch_bounds__(256) void k_scene_e_partial(const float* __restrict__ epart,
+                                                         const int64_t* __restrict__ range, int chunks,
+                                                         double* __restrict__ partial) {
+    __shared__ double part[256];
+    const int sc = int(blockIdx.y);
+    const int64_t lo = range[2 * sc] + int64_t(blockIdx.x) * kEChunk;
+    const int64_t hi = min(range[2 * sc + 1], lo + int64_t(kEChunk));
+    double acc = 0.0;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += 256) acc += double(epart[j]);
+    part[threadIdx.x] = acc;
+    tree_sum_256(part);
+    if (threadIdx.x == 0) partial[int64_t(sc) * chunks + blockIdx.x] = part[0];
+}
+
+// One workgroup per scene: its used chunks, ceil((hi - lo) / kEChunk), in order.
+__global__ __launch_bounds__(256) void k_scene_e_final(const double* __restrict__ partial,
+                                                       const int64_t* __restrict__ range, int chunks,
+                                                       double* __restrict__ grad_E) {
+    __shared__ double part[256];
+    const int sc = int(blockIdx.x);
+    const int64_t len = range[2 * sc + 1] - range[2 * sc];
+    const int used = int(min(int64_t(chunks), (len + kEChunk - 1) / kEChunk));
+    double acc = 0.0;
+    for (int c = int(threadIdx.x); c < used; c += 256) acc += partial[int64_t(sc) * chunks + c];
+    part[threadIdx.x] = acc;
+    tree_sum_256(part);
     if (threadIdx.x == 0) grad_E[sc] += part[0];
 }
 
@@ -895,9 +927,16 @@ hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (a.p.n > 0 && b.grad_E) k_scene_e_sum<<<unsigned(a.p.nscenes), 256, 0, a.stream>>>(a.keys, b.epart, a.p, b.grad_E);
+    if (a.p.n > 0 && b.grad_E) {
+        k_scene_ranges<<<unsigned((2 * a.p.nscenes + 63) / 64), 64, 0, a.stream>>>(a.keys, a.p, b.erange);
+        k_scene_e_partial<<<dim3(unsigned(b.echunks), unsigned(a.p.nscenes)), 256, 0, a.stream>>>(
+            b.epart, b.erange, b.echunks, b.epartial);
+        k_scene_e_final<<<unsigned(a.p.nscenes), 256, 0, a.stream>>>(b.epartial, b.erange, b.echunks, b.grad_E);
+    }
     return hipGetLastError();
 }
+
+int e_chunks(int64_t n) { return int(std::max<int64_t>(1, (n + kEChunk - 1) / kEChunk)); }
 
 hipError_t launch_permute_channels(const uint32_t* pid, const float* src, int64_t src_stride, float* dst,
                                    int64_t dst_stride, int channels, int64_t n, bool to_particle_order,

@@ -102,6 +102,9 @@ struct nclr_mpm_sim {
     float4* slots_b = nullptr;
     float4* gbar = nullptr;
     float* epart = nullptr;
+    int64_t* erange = nullptr;  // adjoint: per scene, its sorted range
+    double* epartial = nullptr; // adjoint: per scene, partial sums of dL/dE
+    int echunks = 1;
     float* gtmp[2] = {nullptr, nullptr};
     char* tape = nullptr;  // saved states of a backward segment
     int64_t tape_states = 0;
@@ -431,6 +434,13 @@ struct nclr_mpm_sim {
         return MPM_OK;
     }
 
+    int echunks_for_scenes() {
+        int64_t largest = 0;
+        for (const auto& sc : scenes) largest = std::max(largest, sc.n);
+        echunks = nclr_host::e_chunks(largest);
+        return echunks;
+    }
+
     int ensure_adjoint() {
         if (slots_b) return MPM_OK;
         const size_t slot_nodes = size_t(nclr_host::max_items(p.n, p.ntiles)) * nclr_host::region_nodes(dim);
@@ -438,6 +448,8 @@ struct nclr_mpm_sim {
         const size_t gfl = size_t(grad_channels()) * size_t(std::max<int64_t>(p.n, 1));
         if (dmalloc(&slots_b, slot_nodes) != hipSuccess || dmalloc(&gbar, gv_nodes) != hipSuccess ||
             dmalloc(&epart, size_t(std::max<int64_t>(p.n, 1))) != hipSuccess ||
+            dmalloc(&erange, 2 * scenes.size()) != hipSuccess ||
+            dmalloc(&epartial, scenes.size() * size_t(echunks_for_scenes())) != hipSuccess ||
             dmalloc(&gtmp[0], gfl) != hipSuccess || dmalloc(&gtmp[1], gfl) != hipSuccess)
             return fail(MPM_ERR_HIP, "hipMalloc of the adjoint buffers failed (out of device memory?)");
         return MPM_OK;
@@ -450,7 +462,7 @@ struct nclr_mpm_sim {
         RC_TRY(ensure_adjoint());
         if (p.n == 0) return MPM_OK;
         const nclr_host::LaunchArgs a = args(cur);
-        const nclr_host::AdjointArgs b{gout, gin, p.n, slots_b, gbar, epart, grad_E};
+        const nclr_host::AdjointArgs b{gout, gin, p.n, slots_b, gbar, epart, grad_E, erange, epartial, echunks};
         PendingEvent pe{};
         if (opts.timing) RC_TRY(record_start(pe, kBackward));
         RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
@@ -522,7 +534,7 @@ struct nclr_mpm_sim {
         pool.clear();
         void* bufs[] = {state[0], state[1], pid[0],  pid[1],   keys,    perm,      keys_tmp, perm_tmp,
                         hist,     tile_start, tile_end, item_offset, slots, gv, stamps, status,
-                        active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, gtmp[0],
+                        active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
                         gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, p.item_tiles};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
@@ -530,6 +542,8 @@ struct nclr_mpm_sim {
         p.item_tiles = nullptr;
         slots_b = gbar = nullptr;
         epart = gtmp[0] = gtmp[1] = nullptr;
+        erange = nullptr;
+        epartial = nullptr;
         tape = nullptr;
         tape_states = 0;
         state[0] = state[1] = nullptr;

@@ -115,7 +115,13 @@ struct AdjointArgs {
     float4* gbar;           // node tiles * NODES: cotangent of the pre-update grid {momentum, mass}
     float* epart;           // n: per sorted particle, its contribution to dL/dE of its scene
     double* grad_E;         // nscenes: accumulated dL/dE
+    int64_t* erange;        // 2 nscenes: each scene's sorted range [lo, hi)
+    double* epartial;       // nscenes * echunks: partial sums of dL/dE
+    int echunks;            // e_chunks(largest scene)
 };
+
+// Chunks of the fixed-order dL/dE sum for a scene of n particles.
+int e_chunks(int64_t n);
 
 hipError_t launch_g2p_adjoint(const LaunchArgs& a, const AdjointArgs& b);
 hipError_t launch_grid_adjoint(const LaunchArgs& a, const AdjointArgs& b);
