@@ -247,10 +247,11 @@ NCLR_DI float ldg(const float* __restrict__ g, int64_t gstride, int c, int64_t j
 
 // ---------------------------------------------------------------------------------------------
 // G2P adjoint (src/nclr.h:167-258).  GENERAL: any material (the scene's, at run time); otherwise
-// every scene is jelly.
+// every scene is jelly.  Register-limited to two waves per SIMD (at most 256 VGPRs; a few spills
+// in 3D): measured 1.45x faster than one wave per SIMD with ~270 VGPRs and no spills.
 // ---------------------------------------------------------------------------------------------
 template <int D, int PPT, bool GENERAL, bool EXACT>
-__global__ __launch_bounds__(256) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
                                                      const uint32_t* __restrict__ tile_start,
                                                      const uint32_t* __restrict__ tile_end,
                                                      const uint32_t* __restrict__ item_offset, StepParams p,
@@ -641,10 +642,11 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------------
-// P2G adjoint (src/nclr.h:104-165, 313-372) per sorted position j
+// P2G adjoint (src/nclr.h:104-165, 313-372) per sorted position j.  Two waves per SIMD, as the
+// G2P adjoint (1.38x faster than one).
 // ---------------------------------------------------------------------------------------------
 template <int D, bool EXACT>
-__global__ __launch_bounds__(256) void k_p2g_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ pid,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p2g_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ pid,
                                                      const uint32_t* __restrict__ perm, StepParams p,
                                                      const float4* __restrict__ gbar, const float* __restrict__ gout,
                                                      float* __restrict__ gin, int64_t gstride,
