@@ -252,7 +252,7 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chu
 
     // tiles: bucket starts and work items
     uint32_t carry_p = 0, carry_i = 0;
-    for (uint32_t b0 = 0; b0 <= p.ntiles; b0 += 4096) {
+    for (uint32_t b0 = 0; b0 < p.ntiles; b0 += 4096) {
         uint32_t cnt[4], its[4], sp_ = 0, si = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -272,8 +272,10 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chu
                 tile_start[tile] = rp;
                 tile_end[tile] = rp + cnt[k];
             }
-            if (tile <= p.ntiles) item_offset[tile] = ri;
-            if (tile < p.ntiles) write_item_tiles(p.item_tiles, ri, its[k], tile);
+            if (tile < p.ntiles) {
+                item_offset[tile] = ri;
+                write_item_tiles(p.item_tiles, ri, its[k], tile);
+            }
             rp += cnt[k];
             ri += its[k];
         }
@@ -281,6 +283,7 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chu
         carry_i += ti;
     }
     if (t == 0) {
+        item_offset[p.ntiles] = carry_i;  // the total
         meta[0] = carry_p;
         meta[1] = 0;  // fill counter of the off-grid bucket
         meta[2] = s_seen;

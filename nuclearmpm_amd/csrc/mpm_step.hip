@@ -81,8 +81,9 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ sta
     __shared__ uint32_t s_placed;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t == 0) s_placed = 0;
+    __syncthreads();
     uint32_t carry = 0, placed = 0;
-    for (uint32_t b = 0; b <= ntiles; b += 4096) {
+    for (uint32_t b = 0; b < ntiles; b += 4096) {
         uint32_t v[4];
         uint32_t s = 0;
 #pragma unroll
@@ -116,13 +117,16 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ sta
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t tile = b + uint32_t(t) * 4 + k;
-            if (tile <= ntiles) item_offset[tile] = run;
-            if (tile < ntiles) write_item_tiles(item_tiles, run, v[k], tile);
+            if (tile < ntiles) {
+                item_offset[tile] = run;
+                write_item_tiles(item_tiles, run, v[k], tile);
+            }
             run += v[k];
         }
         carry += wsum[15];
         __syncthreads();
     }
+    if (t == 0) item_offset[ntiles] = carry;  // the total
     // particles on their grid (sorted first): the incremental sort of the next step reads it
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) placed += __shfl_xor(placed, off);
