@@ -475,32 +475,65 @@ struct nclr_mpm_sim {
         return MPM_OK;
     }
 
-    // Adjoint of `nsteps` steps from the current state (a checkpoint): the states inside the
-    // segment are recomputed into the tape, then the adjoint steps run in reverse.
+    // Adjoint of `nsteps` steps from the current state (a checkpoint).  The states inside the
+    // segment are recomputed straight into the tape: state t (t >= 1) lives in tape entry t - 1,
+    // laid out as save_state writes it, and state 0 stays in the engine's own buffer.  The steps
+    // and the adjoint steps run on those buffers in place (the state pointers are redirected, no
+    // copies); the engine's own buffers, pointers and step counter are restored at the end, so the
+    // state is the checkpoint again.  Graph replay is off inside the segment (graphs hold the
+    // engine's own pointers).
     int backward_segment(int nsteps, const float* gout, float* gin, double* grad_E) {
         RC_TRY(ensure_adjoint());
         if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
         if (p.n == 0) return MPM_OK;
         const size_t sb = state_bytes();
-        if (tape_states < nsteps) {
+        if (tape_states < nsteps - 1) {
             HIP_TRY(hipStreamSynchronize(stream));
             if (tape) (void)hipFree(tape);
             tape = nullptr;
             tape_states = 0;
-            HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tape), sb * size_t(nsteps)));
-            tape_states = nsteps;
+            HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tape), sb * size_t(nsteps - 1)));
+            tape_states = nsteps - 1;
         }
-        const int64_t steps0 = steps;
-        for (int t = 0; t < nsteps; ++t) {
-            RC_TRY(save_state(tape + sb * size_t(t)));
-            if (t + 1 < nsteps) RC_TRY(step());
+        struct Restore {
+            nclr_mpm_sim& e;
+            float* state[2];
+            uint32_t* pid[2];
+            int cur;
+            int64_t steps;
+            int graph;
+            ~Restore() {
+                for (int k = 0; k < 2; ++k) {
+                    e.state[k] = state[k];
+                    e.pid[k] = pid[k];
+                }
+                e.cur = cur;
+                e.steps = steps;
+                e.opts.graph = graph;
+                e.sorted_valid = false;
+            }
+        } restore{*this, {state[0], state[1]}, {pid[0], pid[1]}, cur, steps, opts.graph};
+        opts.graph = 0;
+        auto point_at = [&](int slot, int t) {  // slot <- state t of the segment
+            if (t == 0) {
+                state[slot] = restore.state[restore.cur];
+                pid[slot] = restore.pid[restore.cur];
+            } else {
+                char* entry = tape + sb * size_t(t - 1);
+                state[slot] = reinterpret_cast<float*>(entry);
+                pid[slot] = reinterpret_cast<uint32_t*>(entry + state_floats() * sizeof(float));
+            }
+        };
+        for (int t = 0; t + 1 < nsteps; ++t) {
+            point_at(cur ^ 1, t + 1);
+            RC_TRY(step());
         }
-        steps = steps0;
         const size_t gbytes = size_t(grad_channels()) * size_t(p.n) * sizeof(float);
         HIP_TRY(hipMemcpyAsync(gtmp[0], gout, gbytes, hipMemcpyDeviceToDevice, stream));
         int c = 0;
         for (int t = nsteps - 1; t >= 0; --t) {
-            RC_TRY(load_state(tape + sb * size_t(t)));
+            point_at(cur, t);
+            sorted_valid = false;
             RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E));
             c ^= 1;
         }
