@@ -209,11 +209,13 @@ def test_centre_of_mass_loss_gradient():
 
 
 def test_checkpoint_interval_does_not_change_gradients():
+    """Every checkpoint interval, and graph replay on or off (segments recompute in place into
+    the tape with graphs off and restore the engine's own buffers), gives the same gradients."""
     p = Particles.from_positions(scenes.cube(2, 30, 0.3, 0.5))
     target = torch.tensor([[0.7, 0.3]], dtype=torch.float64, device="cuda")
     grads = []
-    for every in (1, 7, 40):
-        with MPMSimulation(p, MaterialModel.kJelly, res=128, E=5e3) as sim:
+    for every, graph in ((1, False), (7, False), (40, False), (7, True), (40, True)):
+        with MPMSimulation(p, MaterialModel.kJelly, res=128, E=5e3, graph=graph) as sim:
             s0 = to_state(p).requires_grad_(True)
             E = torch.tensor([5e3], dtype=torch.float32, requires_grad=True)
             com_loss(autodiff.rollout(sim, s0, E, 40, every), 2, target, [p.n]).backward()
