@@ -193,11 +193,26 @@ __global__ __launch_bounds__(256) void k_bucket_prefix(StepParams p, const uint3
     if (s == 0) ttotal[k] = total;
 }
 
-// One workgroup of 1024 threads.  On entry meta[0] holds the particles on their grid in the old
-// order (left there by the previous sort).  On exit meta[0] = those of the new order (the start
-// of the off-grid bucket), meta[1] = 0 (the off-grid bucket's fill counter), meta[2] = the old
-// meta[0].  Particles off their grid take no part in P2G and G2P copies them unchanged, so their
-// order within the last bucket cannot affect any result; they are placed by an atomic counter.
+// Bucket starts (exclusive scan of the tile totals) and the P2G work items of the new order.  On
+// entry meta[0] holds the particles on their grid in the old order (left there by the previous
+// sort).  On exit meta[0] = those of the new order (the start of the off-grid bucket), meta[1] = 0
+// (the off-grid bucket's fill counter), meta[2] = the old meta[0].  Particles off their grid take
+// no part in P2G and G2P copies them unchanged, so their order within the last bucket cannot
+// affect any result; they are placed by an atomic counter.
+//
+// Up to kScanRound tiles: k_bucket_scan, one workgroup.  More: k_bucket_round_totals and
+// k_bucket_rounds, one workgroup per round of kScanRound tiles (totals, then each round with the
+// sum of the earlier rounds' totals as its carry); the results are the same.
+struct BucketVisit {
+    uint32_t *tile_start, *tile_end, *item_offset, *item_tiles;
+    NCLR_DI void operator()(uint32_t tile, uint32_t first_p, uint32_t c, uint32_t first_i, uint32_t its) const {
+        tile_start[tile] = first_p;
+        tile_end[tile] = first_p + c;
+        item_offset[tile] = first_i;
+        write_item_tiles(item_tiles, first_i, its, tile);
+    }
+};
+
 template <int D>
 __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chunk,
                                                       uint32_t* __restrict__ item_offset,
@@ -205,88 +220,53 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chu
                                                       uint32_t* __restrict__ tile_end,
                                                       const uint32_t* __restrict__ ttotal,
                                                       uint32_t* __restrict__ meta) {
-    __shared__ uint32_t wsum[2][16];
     __shared__ uint32_t s_seen;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (t == 0) s_seen = meta[0];
+    if (threadIdx.x == 0) s_seen = meta[0];
     __syncthreads();  // meta is rewritten below
-
-    // two simultaneous exclusive scans per round of 1024 x 4 entries
-    auto scan2 = [&](uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb, uint32_t& ta, uint32_t& tb) {
-        uint32_t xa = a, xb = b;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t ya = __shfl_up(xa, off), yb = __shfl_up(xb, off);
-            if (lane >= off) {
-                xa += ya;
-                xb += yb;
-            }
-        }
-        if (lane == 63) {
-            wsum[0][wave] = xa;
-            wsum[1][wave] = xb;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            uint32_t wa = lane < 16 ? wsum[0][lane] : 0u, wb = lane < 16 ? wsum[1][lane] : 0u;
-#pragma unroll
-            for (int off = 1; off < 16; off <<= 1) {
-                const uint32_t ya = __shfl_up(wa, off), yb = __shfl_up(wb, off);
-                if (lane >= off) {
-                    wa += ya;
-                    wb += yb;
-                }
-            }
-            if (lane < 16) {
-                wsum[0][lane] = wa;
-                wsum[1][lane] = wb;
-            }
-        }
-        __syncthreads();
-        ea = (wave ? wsum[0][wave - 1] : 0u) + xa - a;
-        eb = (wave ? wsum[1][wave - 1] : 0u) + xb - b;
-        ta = wsum[0][15];
-        tb = wsum[1][15];
-        __syncthreads();
-    };
-
-    // tiles: bucket starts and work items
-    uint32_t carry_p = 0, carry_i = 0;
-    for (uint32_t b0 = 0; b0 < p.ntiles; b0 += 4096) {
-        uint32_t cnt[4], its[4], sp_ = 0, si = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t tile = b0 + uint32_t(t) * 4 + k;
-            cnt[k] = tile < p.ntiles ? ttotal[tile] : 0u;
-            its[k] = (cnt[k] + chunk - 1) / chunk;
-            sp_ += cnt[k];
-            si += its[k];
-        }
-        uint32_t ep, ei, tp, ti;
-        scan2(sp_, si, ep, ei, tp, ti);
-        uint32_t rp = carry_p + ep, ri = carry_i + ei;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t tile = b0 + uint32_t(t) * 4 + k;
-            if (tile < p.ntiles) {
-                tile_start[tile] = rp;
-                tile_end[tile] = rp + cnt[k];
-            }
-            if (tile < p.ntiles) {
-                item_offset[tile] = ri;
-                write_item_tiles(p.item_tiles, ri, its[k], tile);
-            }
-            rp += cnt[k];
-            ri += its[k];
-        }
-        carry_p += tp;
-        carry_i += ti;
+    const BucketVisit visit{tile_start, tile_end, item_offset, p.item_tiles};
+    auto count = [&](uint32_t tile) { return ttotal[tile]; };
+    uint32_t cp = 0, ci = 0;
+    for (uint32_t b0 = 0; b0 < p.ntiles; b0 += kScanRound) {
+        uint32_t tp, ti;
+        scan_round(b0, p.ntiles, chunk, cp, ci, count, visit, tp, ti);
+        cp += tp;
+        ci += ti;
     }
-    if (t == 0) {
-        item_offset[p.ntiles] = carry_i;  // the total
-        meta[0] = carry_p;
-        meta[1] = 0;  // fill counter of the off-grid bucket
+    if (threadIdx.x == 0) {
+        item_offset[p.ntiles] = ci;  // the total
+        meta[0] = cp;
+        meta[1] = 0;
         meta[2] = s_seen;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bucket_round_totals(uint32_t ntiles, uint32_t chunk,
+                                                              const uint32_t* __restrict__ ttotal,
+                                                              uint32_t* __restrict__ rtot, uint32_t* __restrict__ meta) {
+    uint32_t tp, ti;
+    round_totals(blockIdx.x, ntiles, chunk, [&](uint32_t tile) { return ttotal[tile]; }, tp, ti);
+    if (threadIdx.x == 0) {
+        rtot[2 * blockIdx.x] = tp;
+        rtot[2 * blockIdx.x + 1] = ti;
+        if (blockIdx.x == 0) meta[2] = meta[0];
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(1024) void k_bucket_rounds(StepParams p, uint32_t chunk, uint32_t* __restrict__ item_offset,
+                                                        uint32_t* __restrict__ tile_start,
+                                                        uint32_t* __restrict__ tile_end,
+                                                        const uint32_t* __restrict__ ttotal,
+                                                        const uint32_t* __restrict__ rtot, uint32_t* __restrict__ meta) {
+    uint32_t cp, ci, tp, ti;
+    round_carries(rtot, blockIdx.x, cp, ci);
+    const BucketVisit visit{tile_start, tile_end, item_offset, p.item_tiles};
+    scan_round(blockIdx.x * kScanRound, p.ntiles, chunk, cp, ci, [&](uint32_t tile) { return ttotal[tile]; }, visit,
+               tp, ti);
+    if (threadIdx.x == 0 && blockIdx.x + 1 == gridDim.x) {
+        item_offset[p.ntiles] = ci + ti;
+        meta[0] = cp + tp;
+        meta[1] = 0;
     }
 }
 
@@ -338,8 +318,15 @@ static hipError_t bucket_dispatch(const LaunchArgs& a) {
     if (p.ntiles > 0)
         k_bucket_prefix<D><<<unsigned((int64_t(p.ntiles) + 3) / 4), 256, 0, s>>>(p, a.item_offset, a.ucnt, a.upref,
                                                                                 a.ttotal);
-    k_bucket_scan<D><<<1, 1024, 0, s>>>(p, uint32_t(p.chunk), a.item_offset, a.tile_start, a.tile_end, a.ttotal,
-                                        a.meta);
+    const uint32_t rounds = (p.ntiles + kScanRound - 1) / kScanRound;
+    if (rounds <= 1) {
+        k_bucket_scan<D><<<1, 1024, 0, s>>>(p, uint32_t(p.chunk), a.item_offset, a.tile_start, a.tile_end, a.ttotal,
+                                            a.meta);
+    } else {
+        k_bucket_round_totals<<<rounds, 1024, 0, s>>>(p.ntiles, uint32_t(p.chunk), a.ttotal, a.rtot, a.meta);
+        k_bucket_rounds<D><<<rounds, 1024, 0, s>>>(p, uint32_t(p.chunk), a.item_offset, a.tile_start, a.tile_end,
+                                                   a.ttotal, a.rtot, a.meta);
+    }
     if (p.n > 0)
         k_bucket_scatter<D><<<unsigned((p.n + 255) / 256), 256, 0, s>>>(p, a.keys_tmp, a.perm_tmp, a.uidx, a.upref,
                                                                         a.tile_start, a.meta, a.perm, a.keys,

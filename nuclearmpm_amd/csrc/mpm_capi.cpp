@@ -97,6 +97,7 @@ struct nclr_mpm_sim {
     // incremental sort: valid while the state is stored in the order of the last sort and the
     // tile bounds / work items of that sort are intact (cleared by uploads, loads, re-tuning)
     uint32_t *uidx = nullptr, *ucnt = nullptr, *upref = nullptr, *ttotal = nullptr, *meta = nullptr;
+    uint32_t* rtot = nullptr;
     bool sorted_valid = false;
     // adjoint buffers, allocated by the first backward call
     float4* slots_b = nullptr;
@@ -145,6 +146,7 @@ struct nclr_mpm_sim {
         a.upref = upref;
         a.ttotal = ttotal;
         a.meta = meta;
+        a.rtot = rtot;
         a.stream = stream;
         return a;
     }
@@ -568,10 +570,10 @@ struct nclr_mpm_sim {
         void* bufs[] = {state[0], state[1], pid[0],  pid[1],   keys,    perm,      keys_tmp, perm_tmp,
                         hist,     tile_start, tile_end, item_offset, slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
-                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, p.item_tiles};
+                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, rtot, p.item_tiles};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
-        uidx = ucnt = upref = ttotal = meta = nullptr;
+        uidx = ucnt = upref = ttotal = meta = rtot = nullptr;
         p.item_tiles = nullptr;
         slots_b = gbar = nullptr;
         epart = gtmp[0] = gtmp[1] = nullptr;
@@ -699,6 +701,7 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
         dmalloc(&h->d_scenes, size_t(nscenes)) != hipSuccess || dmalloc(&h->uidx, nn) != hipSuccess ||
         dmalloc(&h->ucnt, bucket_cells) != hipSuccess || dmalloc(&h->upref, bucket_cells) != hipSuccess ||
         dmalloc(&h->ttotal, nbins) != hipSuccess || dmalloc(&h->meta, 4) != hipSuccess ||
+        dmalloc(&h->rtot, 2 * (size_t(tiles) / nclr_dev::kScanRound + 2)) != hipSuccess ||
         dmalloc(&h->p.item_tiles, size_t(nclr_host::max_items(n, uint32_t(tiles)))) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
     p.scenes = h->d_scenes;

@@ -202,6 +202,103 @@ NCLR_DI void write_item_tiles(uint32_t* __restrict__ item_tiles, uint32_t first,
     for (uint32_t c = 0; c < count; ++c) item_tiles[first + c] = tile;
 }
 
+// Scans over tiles, in rounds of kScanRound tiles by a workgroup of 1024 threads (4 consecutive
+// tiles per thread, coalesced).  A tile's particle count c gives ceil(c / chunk) P2G work items.
+
+// Workgroup-wide exclusive scans of two values per thread: ea / eb the thread's prefixes, ta / tb
+// the workgroup's totals.
+NCLR_DI void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb, uint32_t& ta, uint32_t& tb) {
+    __shared__ uint32_t wsum[2][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t xa = a, xb = b;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ya = __shfl_up(xa, off), yb = __shfl_up(xb, off);
+        if (lane >= off) {
+            xa += ya;
+            xb += yb;
+        }
+    }
+    if (lane == 63) {
+        wsum[0][wave] = xa;
+        wsum[1][wave] = xb;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t wa = lane < 16 ? wsum[0][lane] : 0u, wb = lane < 16 ? wsum[1][lane] : 0u;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const uint32_t ya = __shfl_up(wa, off), yb = __shfl_up(wb, off);
+            if (lane >= off) {
+                wa += ya;
+                wb += yb;
+            }
+        }
+        if (lane < 16) {
+            wsum[0][lane] = wa;
+            wsum[1][lane] = wb;
+        }
+    }
+    __syncthreads();
+    ea = (wave ? wsum[0][wave - 1] : 0u) + xa - a;
+    eb = (wave ? wsum[1][wave - 1] : 0u) + xb - b;
+    ta = wsum[0][15];
+    tb = wsum[1][15];
+    __syncthreads();  // wsum is reused by the next call
+}
+
+// Round [b0, b0 + kScanRound) of the tile scans with carries (cp, ci) in: visit(tile,
+// first_particle, count, first_item, items) for each tile below ntiles; tp / ti: the round's totals.
+template <typename Count, typename Visit>
+NCLR_DI void scan_round(uint32_t b0, uint32_t ntiles, uint32_t chunk, uint32_t cp, uint32_t ci, Count count,
+                        Visit visit, uint32_t& tp, uint32_t& ti) {
+    uint32_t cnt[4], its[4], sp = 0, si = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t tile = b0 + uint32_t(threadIdx.x) * 4 + k;
+        cnt[k] = tile < ntiles ? count(tile) : 0u;
+        its[k] = (cnt[k] + chunk - 1) / chunk;
+        sp += cnt[k];
+        si += its[k];
+    }
+    uint32_t ep, ei;
+    block_scan2(sp, si, ep, ei, tp, ti);
+    uint32_t rp = cp + ep, ri = ci + ei;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t tile = b0 + uint32_t(threadIdx.x) * 4 + k;
+        if (tile < ntiles) visit(tile, rp, cnt[k], ri, its[k]);
+        rp += cnt[k];
+        ri += its[k];
+    }
+}
+
+// Totals of round `round` (the first pass of a scan with one workgroup per round).
+template <typename Count>
+NCLR_DI void round_totals(uint32_t round, uint32_t ntiles, uint32_t chunk, Count count, uint32_t& tp, uint32_t& ti) {
+    uint32_t sp = 0, si = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t tile = round * kScanRound + uint32_t(threadIdx.x) * 4 + k;
+        const uint32_t c = tile < ntiles ? count(tile) : 0u;
+        sp += c;
+        si += (c + chunk - 1) / chunk;
+    }
+    uint32_t ep, ei;
+    block_scan2(sp, si, ep, ei, tp, ti);
+}
+
+// Carries into round `round`: the sums of the totals of rounds 0 .. round-1 (rtot[2 r], rtot[2 r + 1]).
+NCLR_DI void round_carries(const uint32_t* __restrict__ rtot, uint32_t round, uint32_t& cp, uint32_t& ci) {
+    uint32_t a = 0, b = 0;
+    for (uint32_t r = threadIdx.x; r < round; r += blockDim.x) {
+        a += rtot[2 * r];
+        b += rtot[2 * r + 1];
+    }
+    uint32_t ea, eb;
+    block_scan2(a, b, ea, eb, cp, ci);
+}
+
 // Exact LDS reductions in fixed point.  A workgroup bounds the sum of |contributions| of every
 // channel (`total`) and scales the channel by 2^e with total * 2^e <= 2^61: each quantised
 // contribution is below 2^62 in magnitude and the 64-bit integer sum is exact.  The quantum,

@@ -11,6 +11,8 @@
 namespace nclr_dev {
 
 constexpr uint32_t kStatusOutOfRange = 1u;
+// Tiles per round of the tile scans (one workgroup of 1024 threads, 4 tiles per thread).
+constexpr uint32_t kScanRound = 4096;
 constexpr uint32_t kStatusMovedTooFar = 2u;  // a particle crossed more than one tile in a step
 constexpr int kMixed = 3;  // material template value: per-scene material chosen at run time
 
@@ -102,6 +104,7 @@ struct LaunchArgs {
     uint32_t* upref;           // max_items * offsets: per (old item, offset), earlier particles bound alike
     uint32_t* ttotal;          // ntiles: particles per new tile
     uint32_t* meta;            // 4 counters
+    uint32_t* rtot;            // 2 per round of the tile scans: particle and work-item totals
     hipStream_t stream;
 };
 

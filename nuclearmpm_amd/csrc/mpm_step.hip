@@ -73,66 +73,59 @@ __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict_
 }
 
 // P2G work items: ceil(count / chunk) per tile, exclusive scan into item_offset[0..ntiles]
-// (item_offset[ntiles] = total).  One workgroup of 1024 threads, coalesced rounds of 4096 tiles.
+// (item_offset[ntiles] = total), and meta[0] = the particles on their grid (sorted first), which
+// the incremental sort of the next step reads.  Up to kScanRound tiles: k_items, one workgroup;
+// more: k_items_round_totals and k_items_rounds, one workgroup per round (same results).
+struct ItemsVisit {
+    uint32_t *item_offset, *item_tiles;
+    NCLR_DI void operator()(uint32_t tile, uint32_t, uint32_t, uint32_t first, uint32_t its) const {
+        item_offset[tile] = first;
+        write_item_tiles(item_tiles, first, its, tile);
+    }
+};
+
 __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                 uint32_t ntiles, uint32_t chunk, uint32_t* __restrict__ item_offset,
                                                 uint32_t* __restrict__ item_tiles, uint32_t* __restrict__ meta) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t s_placed;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (t == 0) s_placed = 0;
-    __syncthreads();
-    uint32_t carry = 0, placed = 0;
-    for (uint32_t b = 0; b < ntiles; b += 4096) {
-        uint32_t v[4];
-        uint32_t s = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t tile = b + uint32_t(t) * 4 + k;
-            const uint32_t c = tile < ntiles ? end[tile] - start[tile] : 0u;
-            v[k] = (c + chunk - 1) / chunk;
-            s += v[k];
-            placed += c;
-        }
-        // inclusive scan of s: within each wave by shuffles, then over the 16 wave totals
-        uint32_t x = s;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off);
-            if (lane >= off) x += y;
-        }
-        if (lane == 63) wsum[wave] = x;
-        __syncthreads();
-        if (wave == 0) {
-            uint32_t ws = lane < 16 ? wsum[lane] : 0u;
-#pragma unroll
-            for (int off = 1; off < 16; off <<= 1) {
-                const uint32_t y = __shfl_up(ws, off);
-                if (lane >= off) ws += y;
-            }
-            if (lane < 16) wsum[lane] = ws;
-        }
-        __syncthreads();
-        uint32_t run = carry + (wave ? wsum[wave - 1] : 0u) + x - s;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t tile = b + uint32_t(t) * 4 + k;
-            if (tile < ntiles) {
-                item_offset[tile] = run;
-                write_item_tiles(item_tiles, run, v[k], tile);
-            }
-            run += v[k];
-        }
-        carry += wsum[15];
-        __syncthreads();
+    const ItemsVisit visit{item_offset, item_tiles};
+    auto count = [&](uint32_t tile) { return end[tile] - start[tile]; };
+    uint32_t cp = 0, ci = 0;
+    for (uint32_t b0 = 0; b0 < ntiles; b0 += kScanRound) {
+        uint32_t tp, ti;
+        scan_round(b0, ntiles, chunk, cp, ci, count, visit, tp, ti);
+        cp += tp;
+        ci += ti;
     }
-    if (t == 0) item_offset[ntiles] = carry;  // the total
-    // particles on their grid (sorted first): the incremental sort of the next step reads it
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) placed += __shfl_xor(placed, off);
-    if (lane == 0) atomicAdd(&s_placed, placed);
-    __syncthreads();
-    if (t == 0) meta[0] = s_placed;
+    if (threadIdx.x == 0) {
+        item_offset[ntiles] = ci;
+        meta[0] = cp;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_items_round_totals(const uint32_t* __restrict__ start,
+                                                             const uint32_t* __restrict__ end, uint32_t ntiles,
+                                                             uint32_t chunk, uint32_t* __restrict__ rtot) {
+    uint32_t tp, ti;
+    round_totals(blockIdx.x, ntiles, chunk, [&](uint32_t tile) { return end[tile] - start[tile]; }, tp, ti);
+    if (threadIdx.x == 0) {
+        rtot[2 * blockIdx.x] = tp;
+        rtot[2 * blockIdx.x + 1] = ti;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_items_rounds(const uint32_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ end, uint32_t ntiles,
+                                                       uint32_t chunk, uint32_t* __restrict__ item_offset,
+                                                       uint32_t* __restrict__ item_tiles,
+                                                       const uint32_t* __restrict__ rtot, uint32_t* __restrict__ meta) {
+    uint32_t cp, ci, tp, ti;
+    round_carries(rtot, blockIdx.x, cp, ci);
+    scan_round(blockIdx.x * kScanRound, ntiles, chunk, cp, ci, [&](uint32_t tile) { return end[tile] - start[tile]; },
+               ItemsVisit{item_offset, item_tiles}, tp, ti);
+    if (threadIdx.x == 0 && blockIdx.x + 1 == gridDim.x) {
+        item_offset[ntiles] = ci + ti;
+        meta[0] = cp + tp;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -721,8 +714,15 @@ static hipError_t sort_dispatch(const LaunchArgs& a) {
         if (e != hipSuccess) return e;
         k_tile_bounds<<<blocks_for(p.n, 256), 256, 0, s>>>(a.keys, p.n, a.tile_start, a.tile_end);
     }
-    k_items<<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset, p.item_tiles,
-                               a.meta);
+    const uint32_t rounds = (p.ntiles + kScanRound - 1) / kScanRound;
+    if (rounds <= 1) {
+        k_items<<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset, p.item_tiles,
+                                   a.meta);
+    } else {
+        k_items_round_totals<<<rounds, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.rtot);
+        k_items_rounds<<<rounds, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset,
+                                               p.item_tiles, a.rtot, a.meta);
+    }
     return hipGetLastError();
 }
 
