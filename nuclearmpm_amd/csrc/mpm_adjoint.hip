@@ -261,26 +261,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     using TL = Tile<D>;
     using L = Layout<D>;
     constexpr int REG = TL::REGION;
-    constexpr int CHUNK = 256 * PPT;
     constexpr int K = D == 2 ? 9 : 27;
     __shared__ unsigned long long acc[D * REG];
     __shared__ uint32_t nonfinite[REG];
     __shared__ float s_bound[4][D];
-    __shared__ uint32_t s_tile, s_chunk;
     const uint32_t item = blockIdx.x;
+    const WorkItem wi = work_item(p, item);  // loaded with the count below, used only if valid
     if (item >= item_offset[p.ntiles]) return;
-    if (threadIdx.x == 0) item_tile(p, item_offset, item, s_tile, s_chunk);
     for (int r = threadIdx.x; r < D * REG; r += blockDim.x) acc[r] = 0ull;
     for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = wi.tile;
     const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_tile(p, tile)];
     int torg[D];
     tile_coords<D>(tile - sp.tile_base, sp.nt, torg);
 #pragma unroll
     for (int a = 0; a < D; ++a) torg[a] *= TL::S;
-    const uint32_t j0 = tile_start[tile] + s_chunk * CHUNK;
-    const uint32_t j1 = min(j0 + uint32_t(CHUNK), tile_end[tile]);
+    const uint32_t j0 = wi.j0, j1 = wi.j1;
     const float four_inv_dx = 4.0f * sp.inv_dx;
 
     // pass 1: cotangents of the gathered velocity (vb = v' + dt x') and affine matrix

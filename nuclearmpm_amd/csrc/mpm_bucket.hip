@@ -53,23 +53,21 @@ __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ 
     using B = Buckets<D>;
     __shared__ uint32_t running[B::NO];
     __shared__ uint32_t wave_count[4][B::NO];
-    __shared__ uint32_t s_tile, s_chunk;
     const uint32_t item = blockIdx.x;
     if (item == 0 && threadIdx.x == 0) *stamp = *stamp + 1u;
+    const WorkItem wi = work_item(p, item);  // loaded with the count below, used only if valid
     if (item >= item_offset[p.ntiles]) return;
-    if (threadIdx.x == 0) item_tile(p, item_offset, item, s_tile, s_chunk);
     for (int o = threadIdx.x; o < B::NO; o += blockDim.x) {
         running[o] = 0;
 #pragma unroll
         for (int w = 0; w < 4; ++w) wave_count[w][o] = 0;
     }
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = wi.tile;
     const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_tile(p, tile)];
     int tc[D];
     tile_coords<D>(tile - sp.tile_base, sp.nt, tc);
-    const uint32_t j0 = tile_start[tile] + s_chunk * chunk;
-    const uint32_t j1 = min(j0 + chunk, tile_end[tile]);
+    const uint32_t j0 = wi.j0, j1 = wi.j1;
     const int wave = threadIdx.x >> 6;
     // new keys and offsets of all rounds first (the loads overlap), then the ranking rounds
     constexpr int kRounds = 4;  // work items hold at most 1024 particles
@@ -204,12 +202,14 @@ __global__ __launch_bounds__(256) void k_bucket_prefix(StepParams p, const uint3
 // k_bucket_rounds, one workgroup per round of kScanRound tiles (totals, then each round with the
 // sum of the earlier rounds' totals as its carry); the results are the same.
 struct BucketVisit {
-    uint32_t *tile_start, *tile_end, *item_offset, *item_tiles;
+    uint32_t *tile_start, *tile_end, *item_offset;
+    uint4* items;
+    uint32_t chunk;
     NCLR_DI void operator()(uint32_t tile, uint32_t first_p, uint32_t c, uint32_t first_i, uint32_t its) const {
         tile_start[tile] = first_p;
         tile_end[tile] = first_p + c;
         item_offset[tile] = first_i;
-        write_item_tiles(item_tiles, first_i, its, tile);
+        write_items(items, first_i, its, tile, first_p, c, chunk);
     }
 };
 
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chu
     __shared__ uint32_t s_seen;
     if (threadIdx.x == 0) s_seen = meta[0];
     __syncthreads();  // meta is rewritten below
-    const BucketVisit visit{tile_start, tile_end, item_offset, p.item_tiles};
+    const BucketVisit visit{tile_start, tile_end, item_offset, p.items, chunk};
     auto count = [&](uint32_t tile) { return ttotal[tile]; };
     uint32_t cp = 0, ci = 0;
     for (uint32_t b0 = 0; b0 < p.ntiles; b0 += kScanRound) {
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(1024) void k_bucket_rounds(StepParams p, uint32_t c
                                                         const uint32_t* __restrict__ rtot, uint32_t* __restrict__ meta) {
     uint32_t cp, ci, tp, ti;
     round_carries(rtot, blockIdx.x, cp, ci);
-    const BucketVisit visit{tile_start, tile_end, item_offset, p.item_tiles};
+    const BucketVisit visit{tile_start, tile_end, item_offset, p.items, chunk};
     scan_round(blockIdx.x * kScanRound, p.ntiles, chunk, cp, ci, [&](uint32_t tile) { return ttotal[tile]; }, visit,
                tp, ti);
     if (threadIdx.x == 0 && blockIdx.x + 1 == gridDim.x) {

@@ -570,11 +570,11 @@ struct nclr_mpm_sim {
         void* bufs[] = {state[0], state[1], pid[0],  pid[1],   keys,    perm,      keys_tmp, perm_tmp,
                         hist,     tile_start, tile_end, item_offset, slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
-                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, rtot, p.item_tiles};
+                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, rtot, p.items};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         uidx = ucnt = upref = ttotal = meta = rtot = nullptr;
-        p.item_tiles = nullptr;
+        p.items = nullptr;
         slots_b = gbar = nullptr;
         epart = gtmp[0] = gtmp[1] = nullptr;
         erange = nullptr;
@@ -702,7 +702,7 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
         dmalloc(&h->ucnt, bucket_cells) != hipSuccess || dmalloc(&h->upref, bucket_cells) != hipSuccess ||
         dmalloc(&h->ttotal, nbins) != hipSuccess || dmalloc(&h->meta, 4) != hipSuccess ||
         dmalloc(&h->rtot, 2 * (size_t(tiles) / nclr_dev::kScanRound + 2)) != hipSuccess ||
-        dmalloc(&h->p.item_tiles, size_t(nclr_host::max_items(n, uint32_t(tiles)))) != hipSuccess)
+        dmalloc(&h->p.items, size_t(nclr_host::max_items(n, uint32_t(tiles)))) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
     p.scenes = h->d_scenes;
     p.stamp = h->stamp;

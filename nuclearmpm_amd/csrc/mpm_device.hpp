@@ -190,16 +190,25 @@ NCLR_DI uint32_t xcd_block() {
 }
 inline unsigned xcd_grid(int64_t blocks) { return unsigned((blocks + kXcds - 1) / kXcds * kXcds); }
 
-// Work item -> (tile, chunk within the tile), from the item table written with the item scan.
-NCLR_DI void item_tile(const StepParams& p, const uint32_t* __restrict__ item_offset, uint32_t item, uint32_t& tile,
-                       uint32_t& chunk) {
-    tile = p.item_tiles[item];
-    chunk = item - item_offset[tile];
+// Work item -> its tile and sorted range [j0, j1), from the item table written with the item scan
+// (one load, uniform over the workgroup).
+struct WorkItem {
+    uint32_t tile, j0, j1;
+};
+NCLR_DI WorkItem work_item(const StepParams& p, uint32_t item) {
+    const uint4 w = p.items[item];
+    return {w.x, w.y, w.z};
 }
 
-// Records the tile of items [first, first + count).
-NCLR_DI void write_item_tiles(uint32_t* __restrict__ item_tiles, uint32_t first, uint32_t count, uint32_t tile) {
-    for (uint32_t c = 0; c < count; ++c) item_tiles[first + c] = tile;
+// Records items [first, first + count) of a tile whose particles are the sorted positions
+// [start, start + particles): item k covers [start + k chunk, min(start + (k + 1) chunk, end)).
+NCLR_DI void write_items(uint4* __restrict__ items, uint32_t first, uint32_t count, uint32_t tile, uint32_t start,
+                         uint32_t particles, uint32_t chunk) {
+    const uint32_t end = start + particles;
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t j0 = start + k * chunk;
+        items[first + k] = make_uint4(tile, j0, min(j0 + chunk, end), 0u);
+    }
 }
 
 // Scans over tiles, in rounds of kScanRound tiles by a workgroup of 1024 threads (4 consecutive

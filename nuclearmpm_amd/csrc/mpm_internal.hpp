@@ -67,7 +67,8 @@ struct StepParams {
     int64_t stride;       // floats between consecutive fields of the SoA state
     const SceneParams* scenes;
     const uint32_t* stamp;  // device word: stamp of the current step (incremented by the key kernel)
-    uint32_t* item_tiles;   // max_items: tile of each P2G work item (written with item_offset)
+    uint4* items;           // max_items: per P2G work item {tile, first, end, 0}, its tile and sorted range
+                            // (written with item_offset)
 };
 
 }  // namespace nclr_dev

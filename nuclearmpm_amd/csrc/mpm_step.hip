@@ -77,17 +77,19 @@ __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict_
 // the incremental sort of the next step reads.  Up to kScanRound tiles: k_items, one workgroup;
 // more: k_items_round_totals and k_items_rounds, one workgroup per round (same results).
 struct ItemsVisit {
-    uint32_t *item_offset, *item_tiles;
-    NCLR_DI void operator()(uint32_t tile, uint32_t, uint32_t, uint32_t first, uint32_t its) const {
+    uint32_t* item_offset;
+    uint4* items;
+    uint32_t chunk;
+    NCLR_DI void operator()(uint32_t tile, uint32_t first_p, uint32_t c, uint32_t first, uint32_t its) const {
         item_offset[tile] = first;
-        write_item_tiles(item_tiles, first, its, tile);
+        write_items(items, first, its, tile, first_p, c, chunk);
     }
 };
 
 __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                 uint32_t ntiles, uint32_t chunk, uint32_t* __restrict__ item_offset,
-                                                uint32_t* __restrict__ item_tiles, uint32_t* __restrict__ meta) {
-    const ItemsVisit visit{item_offset, item_tiles};
+                                                uint4* __restrict__ items, uint32_t* __restrict__ meta) {
+    const ItemsVisit visit{item_offset, items, chunk};
     auto count = [&](uint32_t tile) { return end[tile] - start[tile]; };
     uint32_t cp = 0, ci = 0;
     for (uint32_t b0 = 0; b0 < ntiles; b0 += kScanRound) {
@@ -116,12 +118,12 @@ __global__ __launch_bounds__(1024) void k_items_round_totals(const uint32_t* __r
 __global__ __launch_bounds__(1024) void k_items_rounds(const uint32_t* __restrict__ start,
                                                        const uint32_t* __restrict__ end, uint32_t ntiles,
                                                        uint32_t chunk, uint32_t* __restrict__ item_offset,
-                                                       uint32_t* __restrict__ item_tiles,
+                                                       uint4* __restrict__ items,
                                                        const uint32_t* __restrict__ rtot, uint32_t* __restrict__ meta) {
     uint32_t cp, ci, tp, ti;
     round_carries(rtot, blockIdx.x, cp, ci);
     scan_round(blockIdx.x * kScanRound, ntiles, chunk, cp, ci, [&](uint32_t tile) { return end[tile] - start[tile]; },
-               ItemsVisit{item_offset, item_tiles}, tp, ti);
+               ItemsVisit{item_offset, items, chunk}, tp, ti);
     if (threadIdx.x == 0 && blockIdx.x + 1 == gridDim.x) {
         item_offset[ntiles] = ci + ti;
         meta[0] = cp + tp;
@@ -214,28 +216,25 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
     using TL = Tile<D>;
     using L = Layout<D>;
     constexpr int REG = TL::REGION;
-    constexpr int CHUNK = 256 * PPT;
     // LDS accumulators: exact fixed-point sums (64-bit integer atomics are an order of magnitude
     // faster than float atomics in LDS on gfx950, and make the sum independent of lane order)
     __shared__ unsigned long long acc[4 * REG];
     __shared__ uint32_t nonfinite[REG];  // bit c: a non-finite contribution hit channel c
     __shared__ float s_bound[4][4];      // per wave, per channel
-    __shared__ uint32_t s_tile, s_chunk;
     const uint32_t item = blockIdx.x;
+    const WorkItem wi = work_item(p, item);  // loaded with the count below, used only if valid
     if (item >= item_offset[p.ntiles]) return;
-    if (threadIdx.x == 0) item_tile(p, item_offset, item, s_tile, s_chunk);
     for (int r = threadIdx.x; r < 4 * REG; r += blockDim.x) acc[r] = 0ull;
     for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = wi.tile;
     const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_tile(p, tile)];
     const int material = MAT == kMixed ? sp.material : MAT;
     int torg[D];
     tile_coords<D>(tile - sp.tile_base, sp.nt, torg);
 #pragma unroll
     for (int a = 0; a < D; ++a) torg[a] *= TL::S;
-    const uint32_t j0 = tile_start[tile] + s_chunk * CHUNK;
-    const uint32_t j1 = min(j0 + uint32_t(CHUNK), tile_end[tile]);
+    const uint32_t j0 = wi.j0, j1 = wi.j1;
 
     // pass 1: per particle the fused affine matrix, kept in registers, and a bound of the sum of
     // |contributions| per channel: sum_nodes |w| (|m v_c| + 2.5 dx sum_k |A_ck|)
@@ -716,12 +715,12 @@ static hipError_t sort_dispatch(const LaunchArgs& a) {
     }
     const uint32_t rounds = (p.ntiles + kScanRound - 1) / kScanRound;
     if (rounds <= 1) {
-        k_items<<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset, p.item_tiles,
+        k_items<<<1, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset, p.items,
                                    a.meta);
     } else {
         k_items_round_totals<<<rounds, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.rtot);
         k_items_rounds<<<rounds, 1024, 0, s>>>(a.tile_start, a.tile_end, p.ntiles, uint32_t(p.chunk), a.item_offset,
-                                               p.item_tiles, a.rtot, a.meta);
+                                               p.items, a.rtot, a.meta);
     }
     return hipGetLastError();
 }
