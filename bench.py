@@ -36,6 +36,23 @@ P2G_BYTES_PER_NODE = {2: 12, 3: 16}
 G2P_BYTES_PER_NODE = {2: 8, 3: 12}
 GRID_BYTES_PER_NODE = {2: 24, 3: 32}
 
+# Adjoint kernels of a jelly step, algorithmic (minimum) HBM bytes, SoA FP32, d = 2 / 3:
+#   G2P adjoint reads x (d), F (d^2) and the cotangent of x, v, F, C, Jp (2d + 2d^2 + 1), writes
+#       the partial cotangents of x, F, Jp (d + d^2 + 1); per node with mass it reads the updated
+#       grid velocity (d) and writes the grid cotangent (d)
+#   grid adjoint, per node with mass: reads the pre-update grid {momentum, mass} (d + 1) and the
+#       grid cotangent (d), writes the cotangent of {momentum, mass} (d + 1)
+#   P2G adjoint reads x, v, F, C, mass, volume (2d + 2d^2 + 2) and the partial cotangents (d + d^2 + 1),
+#       writes the cotangent of x, v, F, C, Jp (2d + 2d^2 + 1) and dL/dE (1); per node with mass it
+#       reads the cotangent of {momentum, mass} (d + 1)
+def adjoint_bytes(d: int):
+    ng = 2 * d + 2 * d * d + 1
+    return {
+        "g2p_adjoint": (4 * (d + d * d + ng + d + d * d + 1), 4 * 2 * d),
+        "grid_adjoint": (0, 4 * (d + 1 + d + d + 1)),
+        "p2g_adjoint": (4 * (2 * d + 2 * d * d + 2 + d + d * d + 1 + ng + 1), 4 * (d + 1)),
+    }
+
 
 def load_traffic(kernel: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
@@ -193,6 +210,19 @@ def fwd_bwd(which: str):
         autodiff.rollout(sim, x0, E, steps, every)
     torch.cuda.synchronize()
     tf = time.perf_counter() - t0
+    # a separate pass with HIP events per kernel: the adjoint kernels' rooflines
+    sim.set_timing(True)
+    sim.reset_kernel_times()
+    run()
+    sim.synchronize()
+    g_active = sim.active_nodes()
+    kernels = {}
+    for name, (per_particle, per_node) in adjoint_bytes(d).items():
+        avg_ms, launches = sim.kernel_time(name)
+        nbytes = per_particle * n + per_node * g_active
+        gbs = nbytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        kernels[name] = {"avg_ms": avg_ms, "launches": launches, "alg_bytes": nbytes, "achieved_GBs": gbs,
+                         "frac": gbs / HBM_PEAK_GBS}
     sim.close()
     return {
         "workload": desc,
@@ -203,6 +233,8 @@ def fwd_bwd(which: str):
         "forward_only": n * steps / tf,
         "loss": loss,
         "grad_finite": bool(torch.isfinite(g).all()) and bool(torch.isfinite(ge).all()),
+        "adjoint_kernels": kernels,
+        "g_active_end": g_active,
     }
 
 
