@@ -98,6 +98,7 @@ struct nclr_mpm_sim {
     // tile bounds / work items of that sort are intact (cleared by uploads, loads, re-tuning)
     uint32_t *uidx = nullptr, *ucnt = nullptr, *upref = nullptr, *ttotal = nullptr, *meta = nullptr;
     uint32_t* rtot = nullptr;
+    uint32_t *tile_scene = nullptr, *node_tile_scene = nullptr;  // scene maps (batches only)
     bool sorted_valid = false;
     // adjoint buffers, allocated by the first backward call
     float4* slots_b = nullptr;
@@ -570,11 +571,14 @@ struct nclr_mpm_sim {
         void* bufs[] = {state[0], state[1], pid[0],  pid[1],   keys,    perm,      keys_tmp, perm_tmp,
                         hist,     tile_start, tile_end, item_offset, slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
-                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, rtot, p.items};
+                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, rtot, p.items,
+                        tile_scene, node_tile_scene};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         uidx = ucnt = upref = ttotal = meta = rtot = nullptr;
         p.items = nullptr;
+        tile_scene = node_tile_scene = nullptr;
+        p.tile_scene = p.node_tile_scene = nullptr;
         slots_b = gbar = nullptr;
         epart = gtmp[0] = gtmp[1] = nullptr;
         erange = nullptr;
@@ -706,6 +710,23 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
     p.scenes = h->d_scenes;
     p.stamp = h->stamp;
+    if (nscenes > 1) {
+        std::vector<uint32_t> ts(static_cast<size_t>(tiles)), nts(static_cast<size_t>(node_tiles));
+        for (int s = 0; s < nscenes; ++s) {
+            const auto& sp = h->scenes[size_t(s)];
+            const uint64_t t1 = s + 1 < nscenes ? h->scenes[size_t(s) + 1].tile_base : tiles;
+            const uint64_t n1 = s + 1 < nscenes ? h->scenes[size_t(s) + 1].node_tile_base : node_tiles;
+            std::fill(ts.begin() + sp.tile_base, ts.begin() + int64_t(t1), uint32_t(s));
+            std::fill(nts.begin() + sp.node_tile_base, nts.begin() + int64_t(n1), uint32_t(s));
+        }
+        if (dmalloc(&h->tile_scene, ts.size()) != hipSuccess || dmalloc(&h->node_tile_scene, nts.size()) != hipSuccess ||
+            hipMemcpy(h->tile_scene, ts.data(), ts.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->node_tile_scene, nts.data(), nts.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return cleanup(fail(MPM_ERR_HIP, "scene maps: hipMalloc or copy failed"));
+        p.tile_scene = h->tile_scene;
+        p.node_tile_scene = h->node_tile_scene;
+    }
     if (hipMemcpyAsync(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(nclr_dev::SceneParams),
                        hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         nclr_host::launch_zero_f4(h->gv, int64_t(gv_nodes), h->stream) != hipSuccess ||

@@ -41,12 +41,10 @@ NCLR_DI int scene_search(int nscenes, int64_t v, KeyFn key) {
 NCLR_DI int scene_of_particle(const StepParams& p, uint32_t pid) {
     return scene_search(p.nscenes, pid, [&](int s) { return p.scenes[s].particle_base; });
 }
-NCLR_DI int scene_of_tile(const StepParams& p, uint32_t tile) {
-    return scene_search(p.nscenes, tile, [&](int s) { return int64_t(p.scenes[s].tile_base); });
-}
-NCLR_DI int scene_of_node_tile(const StepParams& p, uint32_t nt) {
-    return scene_search(p.nscenes, nt, [&](int s) { return int64_t(p.scenes[s].node_tile_base); });
-}
+// Tiles and node tiles look their scene up in the static maps built at creation (one load
+// instead of a search over the scene table).
+NCLR_DI int scene_of_tile(const StepParams& p, uint32_t tile) { return int(p.tile_scene[tile]); }
+NCLR_DI int scene_of_node_tile(const StepParams& p, uint32_t nt) { return int(p.node_tile_scene[nt]); }
 
 // base = trunc(x*inv_dx - 0.5), fx = x*inv_dx - base (src/nclr.h:115,121), with the stencil
 // nodes base..base+2 required to lie in [0, res] on every axis.

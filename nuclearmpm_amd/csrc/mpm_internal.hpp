@@ -69,6 +69,8 @@ struct StepParams {
     const uint32_t* stamp;  // device word: stamp of the current step (incremented by the key kernel)
     uint4* items;           // max_items: per P2G work item {tile, first, end, 0}, its tile and sorted range
                             // (written with item_offset)
+    const uint32_t* tile_scene;       // ntiles: scene of each tile (batches of more than one scene)
+    const uint32_t* node_tile_scene;  // node_tiles: scene of each node tile (likewise)
 };
 
 }  // namespace nclr_dev
