@@ -644,7 +644,8 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
 // ---------------------------------------------------------------------------------------------
 template <int D, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p2g_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ pid,
-                                                     const uint32_t* __restrict__ perm, StepParams p,
+                                                     const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ keys, StepParams p,
                                                      const float4* __restrict__ gbar, const float* __restrict__ gout,
                                                      float* __restrict__ gin, int64_t gstride,
                                                      float* __restrict__ epart) {
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     if (j >= p.n) return;
     const int64_t i = perm[j];
     const int64_t s = p.stride;
-    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_particle(p, pid[i])];
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_sorted(p, keys, j, pid[i])];
     float x[D], v[D];
     M<D> F, C;
 #pragma unroll
@@ -916,13 +917,13 @@ hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
     if (a.p.n > 0) {
         if (a.dim == 2)
             k_p2g_adjoint<2, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
-                a.state_in, a.pid_in, a.perm, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
+                a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
         else if (a.exact_svd)
             k_p2g_adjoint<3, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
-                a.state_in, a.pid_in, a.perm, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
+                a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
         else
             k_p2g_adjoint<3, false><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
-                a.state_in, a.pid_in, a.perm, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
+                a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

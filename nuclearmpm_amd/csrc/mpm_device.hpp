@@ -45,6 +45,12 @@ NCLR_DI int scene_of_particle(const StepParams& p, uint32_t pid) {
 // instead of a search over the scene table).
 NCLR_DI int scene_of_tile(const StepParams& p, uint32_t tile) { return int(p.tile_scene[tile]); }
 NCLR_DI int scene_of_node_tile(const StepParams& p, uint32_t nt) { return int(p.node_tile_scene[nt]); }
+// The particle at sorted position j: the scene of its tile (keys[j], this step's sort), or for a
+// particle off its grid (key ntiles) a search by its original index `id`.
+NCLR_DI int scene_of_sorted(const StepParams& p, const uint32_t* __restrict__ keys, int64_t j, uint32_t id) {
+    const uint32_t key = keys[j];
+    return key < p.ntiles ? int(p.tile_scene[key]) : scene_of_particle(p, id);
+}
 
 // base = trunc(x*inv_dx - 0.5), fx = x*inv_dx - base (src/nclr.h:115,121), with the stencil
 // nodes base..base+2 required to lie in [0, res] on every axis.

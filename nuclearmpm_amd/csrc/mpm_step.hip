@@ -496,7 +496,8 @@ template <int D, int MAT, bool EXACT>
 __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st, float* __restrict__ out,
                                                     const uint32_t* __restrict__ pid_in,
                                                     uint32_t* __restrict__ pid_out, const uint32_t* __restrict__ perm,
-                                                    StepParams p, const float4* __restrict__ gv) {
+                                                    const uint32_t* __restrict__ keys, StepParams p,
+                                                    const float4* __restrict__ gv) {
     using L = Layout<D>;
     const int64_t j = int64_t(xcd_block()) * blockDim.x + threadIdx.x;
     if (j >= p.n) return;
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
     const int64_t s = p.stride;
     const uint32_t id = pid_in[i];
     pid_out[j] = id;
-    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_particle(p, id)];
+    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_sorted(p, keys, j, id)];
     const int material = MAT == kMixed ? sp.material : MAT;
     float x[D];
     M<D> F;
@@ -763,11 +764,11 @@ static hipError_t g2p_one(const LaunchArgs& a) {
     const int threads = a.g2p_threads > 0 ? a.g2p_threads : 256;
     if (D == 3 && !a.exact_svd) {
         k_g2p_sorted<D, MAT, false><<<xcd_grid(blocks_for(a.p.n, threads)), threads, 0, a.stream>>>(
-            a.state_in, a.state_out, a.pid_in, a.pid_out, a.perm, a.p, a.gv);
+            a.state_in, a.state_out, a.pid_in, a.pid_out, a.perm, a.keys, a.p, a.gv);
         return hipGetLastError();
     }
     k_g2p_sorted<D, MAT, true><<<xcd_grid(blocks_for(a.p.n, threads)), threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in,
-                                                                              a.pid_out, a.perm, a.p, a.gv);
+                                                                              a.pid_out, a.perm, a.keys, a.p, a.gv);
     return hipGetLastError();
 }
 
