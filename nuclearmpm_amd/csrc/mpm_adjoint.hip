@@ -578,17 +578,27 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                 r = r * TL::R + ra;
             }
             if (!inside) continue;
-            for (int it = 0; it < count[c]; ++it) {
-                const int64_t slot = int64_t(first[c] + it) * TL::REGION + r;
-                const float4 s = slots[slot];
+            // items in order (the forward grid kernel's order); two items' loads in flight
+            auto add = [&](const float4 s, const float4 sb) {
                 g.x += s.x;
                 g.y += s.y;
                 g.z += s.z;
                 g.w += s.w;
-                const float4 sb = slots_b[slot];
                 b.x += sb.x;
                 b.y += sb.y;
                 b.z += sb.z;
+            };
+            const int64_t q = int64_t(first[c]) * TL::REGION + r;
+            int it = 0;
+            for (; it + 2 <= count[c]; it += 2) {
+                const int64_t s0 = q + int64_t(it) * TL::REGION, s1 = s0 + TL::REGION;
+                const float4 a0 = slots[s0], b0 = slots_b[s0], a1 = slots[s1], b1 = slots_b[s1];
+                add(a0, b0);
+                add(a1, b1);
+            }
+            if (it < count[c]) {
+                const int64_t s0 = q + int64_t(it) * TL::REGION;
+                add(slots[s0], slots_b[s0]);
             }
         }
         const float mom[3] = {g.x, g.y, g.z};
