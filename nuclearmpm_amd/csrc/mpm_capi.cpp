@@ -109,6 +109,11 @@ struct nclr_mpm_sim {
     int echunks = 1;
     float* gtmp[2] = {nullptr, nullptr};
     char* tape = nullptr;  // saved states of a backward segment
+    // The sort's outputs (keys, perm, tile bounds, item offsets, the item table, meta) live in one
+    // block, so a backward segment can keep each recomputed step's sort with one copy.
+    char* sort_block = nullptr;
+    size_t sort_block_bytes = 0;
+    char* sort_tape = nullptr;  // per recomputed step of a backward segment, its sort block
     int64_t tape_states = 0;
     int g2p_threads = 256;
     int64_t steps = 0;
@@ -444,6 +449,32 @@ struct nclr_mpm_sim {
         return echunks;
     }
 
+    // keys, perm (n each), tile_start, tile_end, item_offset (ntiles + 1 each), the item table and
+    // meta, each 256-byte aligned inside one allocation
+    hipError_t alloc_sort_block(size_t nn, size_t nbins, size_t max_items) {
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t o = off;
+            off += (bytes + 255) / 256 * 256;
+            return o;
+        };
+        const size_t o_keys = take(nn * sizeof(uint32_t)), o_perm = take(nn * sizeof(uint32_t));
+        const size_t o_ts = take(nbins * sizeof(uint32_t)), o_te = take(nbins * sizeof(uint32_t));
+        const size_t o_io = take(nbins * sizeof(uint32_t)), o_items = take(max_items * sizeof(uint4));
+        const size_t o_meta = take(4 * sizeof(uint32_t));
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&sort_block), off);
+        if (e != hipSuccess) return e;
+        sort_block_bytes = off;
+        keys = reinterpret_cast<uint32_t*>(sort_block + o_keys);
+        perm = reinterpret_cast<uint32_t*>(sort_block + o_perm);
+        tile_start = reinterpret_cast<uint32_t*>(sort_block + o_ts);
+        tile_end = reinterpret_cast<uint32_t*>(sort_block + o_te);
+        item_offset = reinterpret_cast<uint32_t*>(sort_block + o_io);
+        p.items = reinterpret_cast<uint4*>(sort_block + o_items);
+        meta = reinterpret_cast<uint32_t*>(sort_block + o_meta);
+        return hipSuccess;
+    }
+
     int ensure_adjoint() {
         if (slots_b) return MPM_OK;
         const size_t slot_nodes = size_t(nclr_host::max_items(p.n, p.ntiles)) * nclr_host::region_nodes(dim);
@@ -461,14 +492,19 @@ struct nclr_mpm_sim {
 
     // One adjoint step of the current state: recompute sort, P2G and the grid update, then run the
     // adjoint kernels.  The state and the step counter are left unchanged.
-    int backward_step(const float* gout, float* gin, double* grad_E) {
+    // presorted: the sort block already holds this state's sort (a backward segment restored it);
+    // only the device step stamp advances.
+    int backward_step(const float* gout, float* gin, double* grad_E, bool presorted = false) {
         RC_TRY(ensure_adjoint());
         if (p.n == 0) return MPM_OK;
         const nclr_host::LaunchArgs a = args(cur);
         const nclr_host::AdjointArgs b{gout, gin, p.n, slots_b, gbar, epart, grad_E, erange, epartial, echunks};
         PendingEvent pe{};
         if (opts.timing) RC_TRY(record_start(pe, kBackward));
-        RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
+        if (presorted)
+            HIP_TRY(nclr_host::launch_bump_u32(stamp, stream));
+        else
+            RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
         RC_TRY(timed(kP2G, [&] { return nclr_host::launch_p2g(a); }));
         RC_TRY(timed(kGrid, [&] { return nclr_host::launch_grid(a); }));
         RC_TRY(timed(kG2PAdj, [&] { return nclr_host::launch_g2p_adjoint(a, b); }));
@@ -482,9 +518,10 @@ struct nclr_mpm_sim {
     // segment are recomputed straight into the tape: state t (t >= 1) lives in tape entry t - 1,
     // laid out as save_state writes it, and state 0 stays in the engine's own buffer.  The steps
     // and the adjoint steps run on those buffers in place (the state pointers are redirected, no
-    // copies); the engine's own buffers, pointers and step counter are restored at the end, so the
-    // state is the checkpoint again.  Graph replay is off inside the segment (graphs hold the
-    // engine's own pointers).
+    // copies).  Each recomputed step's sort block is kept in the sort tape and restored for that
+    // step's adjoint instead of sorting again.  The engine's own buffers, pointers and step counter
+    // are restored at the end, so the state is the checkpoint again.  Graph replay is off inside
+    // the segment (graphs hold the engine's own pointers).
     int backward_segment(int nsteps, const float* gout, float* gin, double* grad_E) {
         RC_TRY(ensure_adjoint());
         if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
@@ -493,9 +530,11 @@ struct nclr_mpm_sim {
         if (tape_states < nsteps - 1) {
             HIP_TRY(hipStreamSynchronize(stream));
             if (tape) (void)hipFree(tape);
-            tape = nullptr;
+            if (sort_tape) (void)hipFree(sort_tape);
+            tape = sort_tape = nullptr;
             tape_states = 0;
             HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tape), sb * size_t(nsteps - 1)));
+            HIP_TRY(hipMalloc(reinterpret_cast<void**>(&sort_tape), sort_block_bytes * size_t(nsteps - 1)));
             tape_states = nsteps - 1;
         }
         struct Restore {
@@ -527,9 +566,12 @@ struct nclr_mpm_sim {
                 pid[slot] = reinterpret_cast<uint32_t*>(entry + state_floats() * sizeof(float));
             }
         };
+        // step t sorts state t: its sort block is kept for the adjoint of step t
         for (int t = 0; t + 1 < nsteps; ++t) {
             point_at(cur ^ 1, t + 1);
             RC_TRY(step());
+            HIP_TRY(hipMemcpyAsync(sort_tape + sort_block_bytes * size_t(t), sort_block, sort_block_bytes,
+                                   hipMemcpyDeviceToDevice, stream));
         }
         const size_t gbytes = size_t(grad_channels()) * size_t(p.n) * sizeof(float);
         HIP_TRY(hipMemcpyAsync(gtmp[0], gout, gbytes, hipMemcpyDeviceToDevice, stream));
@@ -537,7 +579,11 @@ struct nclr_mpm_sim {
         for (int t = nsteps - 1; t >= 0; --t) {
             point_at(cur, t);
             sorted_valid = false;
-            RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E));
+            const bool kept = t + 1 < nsteps;  // the last state was never stepped, so never sorted
+            if (kept)
+                HIP_TRY(hipMemcpyAsync(sort_block, sort_tape + sort_block_bytes * size_t(t), sort_block_bytes,
+                                       hipMemcpyDeviceToDevice, stream));
+            RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E, kept));
             c ^= 1;
         }
         HIP_TRY(hipMemcpyAsync(gin, gtmp[c], gbytes, hipMemcpyDeviceToDevice, stream));
@@ -568,10 +614,10 @@ struct nclr_mpm_sim {
         drop_graphs();
         for (auto e : pool) (void)hipEventDestroy(e);
         pool.clear();
-        void* bufs[] = {state[0], state[1], pid[0],  pid[1],   keys,    perm,      keys_tmp, perm_tmp,
-                        hist,     tile_start, tile_end, item_offset, slots, gv, stamps, status,
+        void* bufs[] = {state[0], state[1], pid[0],  pid[1],   sort_block, keys_tmp, perm_tmp,
+                        hist,     slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
-                        gtmp[1],  tape,       uidx,     ucnt,        upref, ttotal, meta, rtot, p.items,
+                        gtmp[1],  tape,       sort_tape, uidx,     ucnt,        upref, ttotal, rtot,
                         tile_scene, node_tile_scene};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
@@ -584,6 +630,7 @@ struct nclr_mpm_sim {
         erange = nullptr;
         epartial = nullptr;
         tape = nullptr;
+        sort_block = sort_tape = nullptr;
         tape_states = 0;
         state[0] = state[1] = nullptr;
         pid[0] = pid[1] = nullptr;
@@ -694,19 +741,17 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
         size_t(nclr_host::max_items(n, uint32_t(tiles))) * size_t(nclr_host::bucket_offsets(dim));
     if (dmalloc(&h->state[0], state_floats) != hipSuccess || dmalloc(&h->state[1], state_floats) != hipSuccess ||
         dmalloc(&h->pid[0], size_t(p.stride)) != hipSuccess || dmalloc(&h->pid[1], size_t(p.stride)) != hipSuccess ||
-        dmalloc(&h->keys, nn) != hipSuccess || dmalloc(&h->perm, nn) != hipSuccess ||
+        h->alloc_sort_block(nn, nbins, size_t(nclr_host::max_items(n, uint32_t(tiles)))) != hipSuccess ||
         dmalloc(&h->keys_tmp, nn) != hipSuccess || dmalloc(&h->perm_tmp, nn) != hipSuccess ||
         dmalloc(&h->hist, size_t(256) * size_t(nclr_host::sort_tiles(int64_t(nn))) + 256) != hipSuccess ||
-        dmalloc(&h->tile_start, nbins) != hipSuccess || dmalloc(&h->tile_end, nbins) != hipSuccess ||
-        dmalloc(&h->item_offset, nbins) != hipSuccess || dmalloc(&h->slots, slot_nodes) != hipSuccess ||
+        dmalloc(&h->slots, slot_nodes) != hipSuccess ||
         dmalloc(&h->gv, gv_nodes) != hipSuccess || dmalloc(&h->stamps, size_t(node_tiles)) != hipSuccess ||
         dmalloc(&h->status, 1) != hipSuccess || dmalloc(&h->active, size_t(node_tiles)) != hipSuccess ||
         dmalloc(&h->active_sum, 1) != hipSuccess || dmalloc(&h->stamp, 1) != hipSuccess ||
         dmalloc(&h->d_scenes, size_t(nscenes)) != hipSuccess || dmalloc(&h->uidx, nn) != hipSuccess ||
         dmalloc(&h->ucnt, bucket_cells) != hipSuccess || dmalloc(&h->upref, bucket_cells) != hipSuccess ||
-        dmalloc(&h->ttotal, nbins) != hipSuccess || dmalloc(&h->meta, 4) != hipSuccess ||
-        dmalloc(&h->rtot, 2 * (size_t(tiles) / nclr_dev::kScanRound + 2)) != hipSuccess ||
-        dmalloc(&h->p.items, size_t(nclr_host::max_items(n, uint32_t(tiles)))) != hipSuccess)
+        dmalloc(&h->ttotal, nbins) != hipSuccess ||
+        dmalloc(&h->rtot, 2 * (size_t(tiles) / nclr_dev::kScanRound + 2)) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
     p.scenes = h->d_scenes;
     p.stamp = h->stamp;

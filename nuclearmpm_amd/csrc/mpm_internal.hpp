@@ -148,6 +148,7 @@ hipError_t launch_export_grid(const LaunchArgs& a, int scene, int res, uint32_t 
 hipError_t launch_svd(int dim, bool exact, const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t s);
 hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
+hipError_t launch_bump_u32(uint32_t* p, hipStream_t s);  // *p += 1 on the device
 hipError_t launch_zero_f4(float4* p, int64_t n, hipStream_t s);
 hipError_t launch_active_sum(const LaunchArgs& a, uint32_t first_tile, uint32_t tiles, unsigned long long* out);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* hist,

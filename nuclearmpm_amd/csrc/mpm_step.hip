@@ -689,6 +689,12 @@ __global__ void k_fill_f4(float4* __restrict__ p, int64_t n) {
 // ---------------------------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------------------------
+namespace nclr_dev {
+
+__global__ void k_bump_u32(uint32_t* __restrict__ p) { *p = *p + 1u; }
+
+}  // namespace nclr_dev
+
 namespace nclr_host {
 
 using namespace nclr_dev;
@@ -826,6 +832,11 @@ hipError_t launch_svd(int dim, bool exact, const float* A, float* U, float* S, f
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     k_fill_u32<<<blocks_for(n, 256), 256, 0, s>>>(p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_bump_u32(uint32_t* p, hipStream_t s) {
+    k_bump_u32<<<1, 1, 0, s>>>(p);
     return hipGetLastError();
 }
 
