@@ -200,9 +200,9 @@ int mpm_grad_to_particle_order(mpm_handle h, const float* src, float* dst);
 int mpm_backward_step(mpm_handle h, const float* grad_out, float* grad_in, double* grad_E);
 
 /* Vector-Jacobian product of `nsteps` steps from the current state (a checkpoint): recomputes
- * the states inside the segment on the device into a tape of nsteps - 1 states (device memory,
- * allocated on first use and grown to the longest segment), then runs nsteps backward steps in
- * reverse on the tape.  grad_out is in the slot order of the state nsteps steps ahead, grad_in in
+ * the states inside the segment on the device into a tape of nsteps - 1 states and their sorts
+ * (device memory, allocated on first use and grown to the longest segment), then runs nsteps
+ * backward steps in reverse on the tape.  grad_out is in the slot order of the state nsteps steps ahead, grad_in in
  * the current slot order (they may alias).  The current state is left as it was. */
 int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float* grad_in, double* grad_E);
 
