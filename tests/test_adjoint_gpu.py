@@ -213,22 +213,29 @@ def test_checkpoint_interval_does_not_change_gradients():
     the tape with graphs off and restore the engine's own buffers), gives the same gradients."""
     p = Particles.from_positions(scenes.cube(2, 30, 0.3, 0.5))
     target = torch.tensor([[0.7, 0.3]], dtype=torch.float64, device="cuda")
-    grads, after = [], []
+    grads, after, grids = [], [], []
     for every, graph in ((1, False), (7, False), (40, False), (7, True), (40, True)):
         with MPMSimulation(p, MaterialModel.kJelly, res=128, E=5e3, graph=graph) as sim:
             s0 = to_state(p).requires_grad_(True)
             E = torch.tensor([5e3], dtype=torch.float32, requires_grad=True)
             com_loss(autodiff.rollout(sim, s0, E, 40, every), 2, target, [p.n]).backward()
             grads.append((s0.grad.cpu().numpy(), float(E.grad)))
-            # the backward pass leaves the engine on its own buffers at the first checkpoint
+            # the backward pass leaves the engine on its own buffers at the first checkpoint, and its
+            # grid export shows the last adjoint step's grid: that of a step from the checkpoint
             np.testing.assert_array_equal(sim.particles().x, p.x)
+            grids.append(sim.grid())
             sim.advance(3)
             after.append(sim.particles().x.copy())
     for g, e in grads[1:]:
         np.testing.assert_array_equal(g, grads[0][0])
         assert e == grads[0][1]
     with MPMSimulation(p, MaterialModel.kJelly, res=128, E=5e3) as fresh:
-        fresh.advance(3)
+        fresh.advance(1)
+        gv, gm = fresh.grid()
+        for v, m in grids:
+            np.testing.assert_array_equal(v, gv)
+            np.testing.assert_array_equal(m, gm)
+        fresh.advance(2)
         for x in after:
             np.testing.assert_array_equal(x, fresh.particles().x)
 
