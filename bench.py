@@ -276,6 +276,15 @@ def scene_generation():
             "host_plus_upload_ms": float(np.median(host_ms)), "bitwise_equal": same}
 
 
+LEGS = {
+    "two_cubes_2d": lambda: two_cubes_2d(),
+    "batched": lambda: batched_vs_sequential(200),
+    "fwd_bwd_2d": lambda: fwd_bwd("jelly_2d_diff"),
+    "fwd_bwd_3d": lambda: fwd_bwd("jelly_3d_batched"),
+    "scene_gen": lambda: scene_generation(),
+}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,11 +302,17 @@ def main() -> int:
                          "or 'auto' to let the engine's autotuner time every shape on the warmed-up state")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM traffic counters)")
+    ap.add_argument("--leg", choices=sorted(LEGS),
+                    help="run only this secondary leg and print its JSON record (for profiling it alone); "
+                         "the headline line is not printed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (batched scenes would "
               "shard by scene without communication). Run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.leg:
+        print(json.dumps({args.leg: LEGS[args.leg]()}), flush=True)
+        return 0
 
     from nuclearmpm_amd import MPMSimulation, scenes
 
@@ -408,12 +423,7 @@ def main() -> int:
         "state_finite": finite,
     }
     if not args.no_extra:
-        legs = (("two_cubes_2d", two_cubes_2d),
-                ("batched", lambda: batched_vs_sequential(200)),
-                ("fwd_bwd_2d", lambda: fwd_bwd("jelly_2d_diff")),
-                ("fwd_bwd_3d", lambda: fwd_bwd("jelly_3d_batched")),
-                ("scene_gen", scene_generation))
-        for name, fn in legs:
+        for name, fn in LEGS.items():
             try:
                 out[name] = fn()
             except Exception as e:  # report the failing leg without losing the headline number
