@@ -252,8 +252,6 @@ NCLR_DI float ldg(const float* __restrict__ g, int64_t gstride, int c, int64_t j
 // ---------------------------------------------------------------------------------------------
 template <int D, int PPT, bool GENERAL, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
-                                                     const uint32_t* __restrict__ tile_start,
-                                                     const uint32_t* __restrict__ tile_end,
                                                      const uint32_t* __restrict__ item_offset, StepParams p,
                                                      const float4* __restrict__ gv, const float* __restrict__ gout,
                                                      float* __restrict__ gin, int64_t gstride,
@@ -886,8 +884,7 @@ template <int D, int PPT, bool GENERAL, bool EXACT>
 static hipError_t g2p_adjoint_one(const LaunchArgs& a, const AdjointArgs& b) {
     const int64_t items = (a.p.n + 256 * PPT - 1) / (256 * PPT) + a.p.ntiles;
     k_g2p_adjoint<D, PPT, GENERAL, EXACT><<<unsigned(items), 256, 0, a.stream>>>(
-        a.state_in, a.perm, a.tile_start, a.tile_end, a.item_offset, a.p, a.gv, b.grad_out, b.grad_in, b.gstride,
-        b.slots_b);
+        a.state_in, a.perm, a.item_offset, a.p, a.gv, b.grad_out, b.grad_in, b.gstride, b.slots_b);
     return hipGetLastError();
 }
 

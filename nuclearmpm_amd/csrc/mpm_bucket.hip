@@ -45,8 +45,6 @@ template <int D>
 __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ st, StepParams p,
                                                       uint32_t* __restrict__ stamp,
                                                       const uint32_t* __restrict__ item_offset,
-                                                      const uint32_t* __restrict__ tile_start,
-                                                      const uint32_t* __restrict__ tile_end, uint32_t chunk,
                                                       uint32_t* __restrict__ newkey, uint32_t* __restrict__ rankpk,
                                                       uint32_t* __restrict__ uidx, uint32_t* __restrict__ ucnt,
                                                       uint32_t* __restrict__ status) {
@@ -313,8 +311,7 @@ static hipError_t bucket_dispatch(const LaunchArgs& a) {
     hipStream_t s = a.stream;
     const int64_t items = max_items(p.n, p.ntiles);
     k_bucket_count<D><<<unsigned(std::max<int64_t>(items, 1)), 256, 0, s>>>(
-        a.state_in, p, a.stamp, a.item_offset, a.tile_start, a.tile_end, uint32_t(p.chunk), a.keys_tmp, a.perm_tmp,
-        a.uidx, a.ucnt, a.status);
+        a.state_in, p, a.stamp, a.item_offset, a.keys_tmp, a.perm_tmp, a.uidx, a.ucnt, a.status);
     if (p.ntiles > 0)
         k_bucket_prefix<D><<<unsigned((int64_t(p.ntiles) + 3) / 4), 256, 0, s>>>(p, a.item_offset, a.ucnt, a.upref,
                                                                                 a.ttotal);

@@ -209,8 +209,6 @@ NCLR_DI void p2g_deposit(unsigned long long* __restrict__ acc, uint32_t* __restr
 }
 template <int D, int MAT, int PPT, bool EXACT>
 __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, const uint32_t* __restrict__ perm,
-                                                  const uint32_t* __restrict__ tile_start,
-                                                  const uint32_t* __restrict__ tile_end,
                                                   const uint32_t* __restrict__ item_offset, StepParams p,
                                                   float4* __restrict__ slots) {
     using TL = Tile<D>;
@@ -737,11 +735,11 @@ static hipError_t p2g_one(const LaunchArgs& a) {
     const int64_t items = (a.p.n + 256 * PPT - 1) / (256 * PPT) + a.p.ntiles;
     if (D == 3 && !a.exact_svd) {
         k_p2g_tile<D, MAT, PPT, false><<<unsigned(items), 256, 0, a.stream>>>(
-            a.state_in, a.perm, a.tile_start, a.tile_end, a.item_offset, a.p, a.slots);
+            a.state_in, a.perm, a.item_offset, a.p, a.slots);
         return hipGetLastError();
     }
-    k_p2g_tile<D, MAT, PPT, true><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.tile_start, a.tile_end,
-                                                                   a.item_offset, a.p, a.slots);
+    k_p2g_tile<D, MAT, PPT, true><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.item_offset, a.p,
+                                                                   a.slots);
     return hipGetLastError();
 }
 
