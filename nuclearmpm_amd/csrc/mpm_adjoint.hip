@@ -828,14 +828,30 @@ NCLR_DI void tree_sum_256(double* part) {
     }
 }
 
+// Scene sc's sorted range: from the range table, or with range == nullptr (a single scene)
+// [0, meta[0]), the particles on the grid, which the sort places first.
+NCLR_DI void scene_range(const int64_t* __restrict__ range, const uint32_t* __restrict__ meta, int sc,
+                         int64_t& lo, int64_t& hi) {
+    if (range) {
+        lo = range[2 * sc];
+        hi = range[2 * sc + 1];
+    } else {
+        lo = 0;
+        hi = meta[0];
+    }
+}
+
 // Grid (chunks, scenes): chunk blockIdx.x of scene blockIdx.y -> partial[scene * chunks + chunk].
 __global__ __launch_bounds__(256) void k_scene_e_partial(const float* __restrict__ epart,
-                                                         const int64_t* __restrict__ range, int chunks,
+                                                         const int64_t* __restrict__ range,
+                                                         const uint32_t* __restrict__ meta, int chunks,
                                                          double* __restrict__ partial) {
     __shared__ double part[256];
     const int sc = int(blockIdx.y);
-    const int64_t lo = range[2 * sc] + int64_t(blockIdx.x) * kEChunk;
-    const int64_t hi = min(range[2 * sc + 1], lo + int64_t(kEChunk));
+    int64_t s_lo, s_hi;
+    scene_range(range, meta, sc, s_lo, s_hi);
+    const int64_t lo = s_lo + int64_t(blockIdx.x) * kEChunk;
+    const int64_t hi = min(s_hi, lo + int64_t(kEChunk));
     double acc = 0.0;
     for (int64_t j = lo + threadIdx.x; j < hi; j += 256) acc += double(epart[j]);
     part[threadIdx.x] = acc;
@@ -845,11 +861,14 @@ __global__ __launch_bounds__(256) void k_scene_e_partial(const float* __restrict
 
 // One workgroup per scene: its used chunks, ceil((hi - lo) / kEChunk), in order.
 __global__ __launch_bounds__(256) void k_scene_e_final(const double* __restrict__ partial,
-                                                       const int64_t* __restrict__ range, int chunks,
+                                                       const int64_t* __restrict__ range,
+                                                       const uint32_t* __restrict__ meta, int chunks,
                                                        double* __restrict__ grad_E) {
     __shared__ double part[256];
     const int sc = int(blockIdx.x);
-    const int64_t len = range[2 * sc + 1] - range[2 * sc];
+    int64_t s_lo, s_hi;
+    scene_range(range, meta, sc, s_lo, s_hi);
+    const int64_t len = s_hi - s_lo;
     const int used = int(min(int64_t(chunks), (len + kEChunk - 1) / kEChunk));
     double acc = 0.0;
     for (int c = int(threadIdx.x); c < used; c += 256) acc += partial[int64_t(sc) * chunks + c];
@@ -935,10 +954,12 @@ hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
         if (e != hipSuccess) return e;
     }
     if (a.p.n > 0 && b.grad_E) {
-        k_scene_ranges<<<unsigned((2 * a.p.nscenes + 63) / 64), 64, 0, a.stream>>>(a.keys, a.p, b.erange);
+        // one scene: its range is [0, meta[0]), no search needed
+        const int64_t* range = a.p.nscenes == 1 ? nullptr : b.erange;
+        if (range) k_scene_ranges<<<unsigned((2 * a.p.nscenes + 63) / 64), 64, 0, a.stream>>>(a.keys, a.p, b.erange);
         k_scene_e_partial<<<dim3(unsigned(b.echunks), unsigned(a.p.nscenes)), 256, 0, a.stream>>>(
-            b.epart, b.erange, b.echunks, b.epartial);
-        k_scene_e_final<<<unsigned(a.p.nscenes), 256, 0, a.stream>>>(b.epartial, b.erange, b.echunks, b.grad_E);
+            b.epart, range, a.meta, b.echunks, b.epartial);
+        k_scene_e_final<<<unsigned(a.p.nscenes), 256, 0, a.stream>>>(b.epartial, range, a.meta, b.echunks, b.grad_E);
     }
     return hipGetLastError();
 }
