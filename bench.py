@@ -55,15 +55,24 @@ def adjoint_bytes(d: int):
 
 
 def load_traffic(kernel: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present, with the state
+    it was measured on: (bytes, {"g_active": nodes with mass then, "alg_bytes": the algorithmic
+    bytes of that launch, "ratio": bytes / alg_bytes, "source": the summary}).  The counters come
+    from their own profiled run, not from this one: the active-node count of the 3D snow workload
+    grows with the step count, so the ratio is the comparable figure."""
     path = os.path.join(HERE, "profiles", f"traffic_{kernel}_four_cubes_3d.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            rec = json.load(f)
     except Exception:
-        return None
+        return None, None
+    b = rec.get("bytes_per_launch")
+    state = {k: rec.get(k) for k in ("g_active", "alg_bytes", "steps_before", "source")}
+    if b and rec.get("alg_bytes"):
+        state["ratio"] = b / rec["alg_bytes"]
+    return b, state
 
 
 def cpu_baseline(sc, steps: int):
@@ -76,7 +85,15 @@ def cpu_baseline(sc, steps: int):
     p = sc.particles
     st = O.State(x=p.x, v=p.v, F=p.F, C=p.C, Jp=p.Jp, mass=p.mass, volume=p.volume)
     secs = O.time_steps(st, int(sc.material), sc.res, sc.dt, sc.E, sc.nu, sc.gravity, warmup=1, steps=steps)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    # every core this process may run on (the GPU box allots 16 to one GPU's job and says so in
+    # OMP_NUM_THREADS); nproc and the affinity mask are recorded beside the count
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = nproc
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(usable, omp) if omp > 0 else usable
     msteps = 2 * steps
     psecs = O.time_steps(st, int(sc.material), sc.res, sc.dt, sc.E, sc.nu, sc.gravity, warmup=1, steps=msteps,
                          threads=threads)
@@ -94,13 +111,14 @@ def cpu_baseline(sc, steps: int):
         "sample": f"same workload ({p.n} particles, res {sc.res}): 1 untimed + {steps} timed steps of the "
                   f"restated serial reference step (oracle/, FP32, -O3 -ffast-math), {secs:.1f} s",
         "cpu_model": model,
+        "host": {"nproc": nproc, "affinity_cpus": usable, "OMP_NUM_THREADS": omp or None},
         "multicore": {
             "value": p.n * msteps / psecs,
             "unit": "particle-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"1 untimed + {msteps} timed steps of the same step on {threads} OpenMP threads (per-thread "
-                      f"P2G grids summed per node), {psecs:.1f} s",
+            "sample": f"1 untimed + {msteps} timed steps of the same step on {threads} OpenMP threads (each thread adds "
+                      f"the contributions to its own band of grid rows, in particle order: bitwise the serial step), {psecs:.1f} s",
         },
     }
 
@@ -127,20 +145,29 @@ def two_cubes_2d():
 
 def batched_vs_sequential(steps: int):
     """2D batched workload: 256 seeded scenes (1-4 cubes each, liquid / snow / jelly, res 128) in
-    one launch sequence per step, against the same scenes advanced one plain simulation at a time;
-    particle-steps/s of each and the ratio."""
+    one launch sequence per step for the configured 2,000 steps, against the same scenes advanced
+    one plain simulation at a time; particle-steps/s of each and the ratio.  A scene whose particle
+    leaves its grid stops at that step on its own (the reference throws out of that scene's
+    advance()); it is reported, and only the steps the scenes actually took are counted."""
     from nuclearmpm_amd import MPMBatch, MPMSimulation, scenes
 
     specs = scenes.batched_2d(seed=1, scenes=256)
-    n = sum(s.particles.n for s in specs)
+    nps = [s.particles.n for s in specs]
+    n = sum(nps)
+    warm = 5
     with MPMBatch(specs) as b:
-        b.advance(5)
+        b.advance(warm)
         b.synchronize()
         t0 = time.perf_counter()
         b.advance(steps)
-        b.synchronize()
+        status = b.scene_status()  # waits for the steps; a stopped scene is not an error here
         tb = time.perf_counter() - t0
-    seq_steps = max(5, steps // 4)
+    live = 0
+    for s, m in enumerate(nps):
+        st = status.get(s)
+        taken = steps if st is None else min(steps, max(0, st["step"] - 1 - warm))
+        live += m * taken
+    seq_steps = 50
     ts = 0.0
     for s in specs:
         with MPMSimulation(s.particles, s.material, res=s.res, dt=s.dt, E=s.E, nu=s.nu, gravity=s.gravity) as sim:
@@ -150,13 +177,16 @@ def batched_vs_sequential(steps: int):
             sim.advance(seq_steps)
             sim.synchronize()
             ts += time.perf_counter() - t0
-    batched = n * steps / tb
+    batched = live / tb
     sequential = n * seq_steps / ts
     return {
         "workload": "batched_2d: 256 scenes (seed 1), 1-4 cubes each, side resolution U{20..80}, "
                     "materials uniform over liquid/snow/jelly, res 128, dt 1e-4",
         "particles": n,
-        "batched": {"value": batched, "unit": "particle-steps/s", "steps": steps},
+        "batched": {"value": batched, "unit": "particle-steps/s", "steps": steps,
+                    "stopped_scenes": {str(s): {"step": st["step"], "particles": nps[s],
+                                                "material": int(specs[s].material)} for s, st in status.items()},
+                    "particle_steps_counted": live},
         "sequential": {"value": sequential, "unit": "particle-steps/s", "steps_per_scene": seq_steps},
         "speedup": batched / sequential,
     }
@@ -278,7 +308,7 @@ def scene_generation():
 
 LEGS = {
     "two_cubes_2d": lambda: two_cubes_2d(),
-    "batched": lambda: batched_vs_sequential(200),
+    "batched": lambda: batched_vs_sequential(2000),
     "fwd_bwd_2d": lambda: fwd_bwd("jelly_2d_diff"),
     "fwd_bwd_3d": lambda: fwd_bwd("jelly_3d_batched"),
     "scene_gen": lambda: scene_generation(),
@@ -382,6 +412,7 @@ def main() -> int:
         v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
     achieved = per_kernel[dom]["achieved_GBs"]
+    traffic, traffic_state = load_traffic(dom)
 
     out = {
         "metric": METRIC,
@@ -416,7 +447,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(dom),
+            "traffic": traffic,
+            "traffic_measured_on": traffic_state,
         },
         "kernels": per_kernel,
         "g_active": g_active,

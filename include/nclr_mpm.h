@@ -101,8 +101,25 @@ int mpm_create_batch(int dim, int nscenes, const mpm_scene* scenes, const float*
 int mpm_advance(mpm_handle h, int nsteps);
 
 /* Waits for queued work; returns MPM_ERR_OUT_OF_RANGE if any step since the last check found a
- * particle outside the grid (the reference throws std::out_of_range from std::vector::at). */
+ * particle outside its scene's grid (the reference throws std::out_of_range from std::vector::at,
+ * src/nclr.h:163-164); the message names the scenes and the step of each failure.
+ * Each scene fails on its own, as each MPMSimulation object of the reference does: the step that
+ * finds one of a scene's particles off its grid leaves that scene's particles as they were before
+ * the step (the state the reference's throwing advance() leaves), and the scene stays stopped;
+ * later steps advance the other scenes and find the stopped scene again (every later advance() of
+ * the reference throws again).  Setting or loading a state (mpm_set_particles, mpm_load_state,
+ * mpm_set_state_device) restarts every scene.  A stopped scene's grid export shows its last
+ * completed step. */
 int mpm_synchronize(mpm_handle h);
+
+/* Per-scene status without raising it as an error (no reference counterpart: a batch of the
+ * reference is one MPMSimulation per scene, each throwing on its own): waits for queued work and
+ * fills, for each scene s, flags[s] = the status bits found since the state was last set
+ * (1: a particle's stencil left the grid, 2: a particle crossed more than one tile) and
+ * failed_step[s] = the step that stopped the scene (1-based count of steps taken) or -1.  Either
+ * pointer may be NULL.  The failures it returns are not reported again by mpm_synchronize until
+ * a later step finds them again. */
+int mpm_scene_status(mpm_handle h, uint32_t* flags, int64_t* failed_step);
 
 /* Replaces MPMSimulation<dim>::particles() (src/nclr.h:86): copies the current particle state of
  * all scenes to host arrays in the original particle order.  Any pointer may be NULL. */
@@ -191,7 +208,9 @@ int mpm_set_state_device(mpm_handle h, const float* src);
 int mpm_grad_from_particle_order(mpm_handle h, const float* src, float* dst);
 int mpm_grad_to_particle_order(mpm_handle h, const float* src, float* dst);
 
-/* Vector-Jacobian product of one step taken from the current state (every material: jelly; snow
+/* The adjoint entry points below return MPM_ERR_OUT_OF_RANGE while a scene is stopped.
+ *
+ * Vector-Jacobian product of one step taken from the current state (every material: jelly; snow
  * with the backward pass of its singular-value clamp, its Jp update and its hardening; liquid):
  * grad_out is the cotangent of the state after the step in ITS slot order,
  * grad_in (must not alias grad_out) receives the cotangent of the current state in the current
@@ -205,6 +224,12 @@ int mpm_backward_step(mpm_handle h, const float* grad_out, float* grad_in, doubl
  * backward steps in reverse on the tape.  grad_out is in the slot order of the state nsteps steps ahead, grad_in in
  * the current slot order (they may alias).  The current state is left as it was. */
 int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float* grad_in, double* grad_E);
+
+/* Device bytes of the tape mpm_backward_segment keeps for segments of `nsteps` steps (-1 for a
+ * NULL handle or nsteps < 1): nsteps - 1 entries of one state snapshot (mpm_state_bytes) and one
+ * sort (8 bytes per particle, 12 bytes per particle tile of every scene for its bounds and work
+ * item offset, 16 bytes per possible work item, 256-byte aligned). */
+int64_t mpm_tape_bytes(mpm_handle h, int nsteps);
 
 /* Sets Young's modulus and Poisson's ratio of one scene (mu_0, lambda_0 recomputed in float as
  * src/nclr.h:77-78 does). */

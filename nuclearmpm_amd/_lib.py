@@ -108,6 +108,8 @@ def lib() -> ctypes.CDLL:
     L.mpm_advance.restype = i32
     L.mpm_synchronize.argtypes = [H]
     L.mpm_synchronize.restype = i32
+    L.mpm_scene_status.argtypes = [H, _vp, _vp]
+    L.mpm_scene_status.restype = i32
     L.mpm_get_particles.argtypes = [H, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.mpm_get_particles.restype = i32
     L.mpm_set_particles.argtypes = [H, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
@@ -148,6 +150,8 @@ def lib() -> ctypes.CDLL:
     L.mpm_backward_step.restype = i32
     L.mpm_backward_segment.argtypes = [H, i32, _vp, _vp, _vp]
     L.mpm_backward_segment.restype = i32
+    L.mpm_tape_bytes.argtypes = [H, i32]
+    L.mpm_tape_bytes.restype = i64
     L.mpm_set_scene_elasticity.argtypes = [H, i32, f32, f32]
     L.mpm_set_scene_elasticity.restype = i32
     u32x4 = ctypes.POINTER(ctypes.c_uint32)

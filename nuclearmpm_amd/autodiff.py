@@ -65,7 +65,8 @@ class _Rollout(torch.autograd.Function):
             raise ValueError("steps and checkpoint_every must be >= 1")
         state0 = state0.contiguous()
         _ready(state0)
-        for s, e in enumerate(E.detach().double().cpu().tolist()):
+        E_vals = E.detach().double().cpu().tolist()
+        for s, e in enumerate(E_vals):
             sim.set_elasticity(s, e)
         check(L.mpm_set_state_device(h, state0.data_ptr()))
         sb = int(L.mpm_state_bytes(h))
@@ -88,6 +89,8 @@ class _Rollout(torch.autograd.Function):
         ctx.ckpts = ckpts
         ctx.final = final
         ctx.E_meta = (E.dtype, E.device, E.shape)
+        ctx.E_vals = E_vals
+        ctx.nu_vals = list(sim._nus)
         return out
 
     @staticmethod
@@ -96,6 +99,10 @@ class _Rollout(torch.autograd.Function):
         L, h = lib(), sim._handle()
         g_user = grad_out.contiguous().float()
         _ready(g_user)
+        # the segment adjoint recomputes the forward with the engine's current elasticity: restore
+        # this rollout's E (another rollout or set_elasticity may have changed it since)
+        for s, (e, nu) in enumerate(zip(ctx.E_vals, ctx.nu_vals)):
+            sim.set_elasticity(s, e, nu)
         g = torch.empty_like(g_user)
         check(L.mpm_load_state(h, ctx.final.data_ptr()))
         check(L.mpm_grad_from_particle_order(h, g_user.data_ptr(), g.data_ptr()))

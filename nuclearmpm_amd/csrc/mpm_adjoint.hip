@@ -387,14 +387,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int b = 0; b < D; ++b) s += fabsf(Cb[k].a[a][b]);
             // |off - fx| <= 1.5 on every axis
             const float bc = wsum * (fabsf(vb[k][a]) + 1.5f * four_inv_dx * s);
-            if (bc < 3.0e38f) bound[a] += bc;
+            if (bc < 3.0e38f) bound[a] = fmaxf(bound[a], bc);
         }
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         float b = bound[c];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off);
+        for (int off = 32; off > 0; off >>= 1) b = fmaxf(b, __shfl_xor(b, off));
         bound[c] = b;
     }
     if ((threadIdx.x & 63) == 0)
@@ -405,7 +405,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     double inv_scale[D];
 #pragma unroll
     for (int c = 0; c < D; ++c)
-        fixed_point_scale((s_bound[0][c] + s_bound[1][c]) + (s_bound[2][c] + s_bound[3][c]), scale[c], inv_scale[c]);
+        fixed_point_scale(fmaxf(fmaxf(s_bound[0][c], s_bound[1][c]), fmaxf(s_bound[2][c], s_bound[3][c])), scale[c],
+                          inv_scale[c]);
 
     // pass 2: gather the updated grid (for the forward C and the weight derivatives), scatter the
     // quantised grid cotangent, write the partial x and F cotangents

@@ -62,7 +62,8 @@ __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ 
     }
     __syncthreads();
     const uint32_t tile = wi.tile;
-    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_tile(p, tile)];
+    const int scene = p.nscenes == 1 ? 0 : scene_of_tile(p, tile);
+    const SceneParams sp = p.scenes[scene];
     int tc[D];
     tile_coords<D>(tile - sp.tile_base, sp.nt, tc);
     const uint32_t j0 = wi.j0, j1 = wi.j1;
@@ -94,10 +95,10 @@ __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ 
                 o_r[r] = offset_index<D>(delta);
                 key_r[r] = sp.tile_base + tile_key<D>(base, sp.nt);
             } else {
-                atomicOr(status, kStatusMovedTooFar);
+                stop_scene(p, scene, kStatusMovedTooFar);
             }
         } else {
-            atomicOr(status, kStatusOutOfRange);
+            stop_scene(p, scene, kStatusOutOfRange);
         }
     }
 #pragma unroll
@@ -274,6 +275,7 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(StepParams p, const uint
                                                         const uint32_t* __restrict__ uidx,
                                                         const uint32_t* __restrict__ upref,
                                                         const uint32_t* __restrict__ tile_start,
+                                                        const uint32_t* __restrict__ pid,
                                                         uint32_t* __restrict__ meta, uint32_t* __restrict__ perm,
                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ status) {
     using B = Buckets<D>;
@@ -289,7 +291,9 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(StepParams p, const uint
     } else {  // off the grid (now or already before the step)
         key = p.ntiles;
         pos = meta[0] + atomicAdd(&meta[1], 1u);
-        if (uint32_t(j) >= seen) atomicOr(status, kStatusOutOfRange);
+        // already off the grid before the step: its scene is reported again (every advance() of the
+        // reference throws for that scene again)
+        if (uint32_t(j) >= seen) stop_scene(p, p.nscenes == 1 ? 0 : scene_of_particle(p, pid[j]), kStatusOutOfRange);
     }
     if (pos < p.n) {
         perm[pos] = uint32_t(j);
@@ -326,8 +330,8 @@ static hipError_t bucket_dispatch(const LaunchArgs& a) {
     }
     if (p.n > 0)
         k_bucket_scatter<D><<<unsigned((p.n + 255) / 256), 256, 0, s>>>(p, a.keys_tmp, a.perm_tmp, a.uidx, a.upref,
-                                                                        a.tile_start, a.meta, a.perm, a.keys,
-                                                                        a.status);
+                                                                        a.tile_start, a.pid_in, a.meta, a.perm,
+                                                                        a.keys, a.status);
     return hipGetLastError();
 }
 

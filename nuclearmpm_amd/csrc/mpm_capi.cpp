@@ -99,6 +99,12 @@ struct nclr_mpm_sim {
     uint32_t *uidx = nullptr, *ucnt = nullptr, *upref = nullptr, *ttotal = nullptr, *meta = nullptr;
     uint32_t* rtot = nullptr;
     uint32_t *tile_scene = nullptr, *node_tile_scene = nullptr;  // scene maps (batches only)
+    // per scene (device): status bits raised since the last read, and the stamp of the step that
+    // stopped the scene (StepParams::scene_flags / scene_frozen); host: every bit reported since
+    // the state was last set
+    uint32_t *scene_flags = nullptr, *scene_frozen = nullptr;
+    std::vector<uint32_t> scene_seen;
+    int64_t stamp_host = 0;  // mirror of the device step stamp (one increment per step and per adjoint step)
     bool sorted_valid = false;
     // adjoint buffers, allocated by the first backward call
     float4* slots_b = nullptr;
@@ -267,28 +273,92 @@ struct nclr_mpm_sim {
         if (p.n > 0) cur ^= 1;
         sorted_valid = true;
         steps += 1;
+        stamp_host += 1;
         return MPM_OK;
     }
 
-    int sync() {
+    // Waits for the stream, checks the engine-wide status word and reads the per-scene words;
+    // the scenes' new status bits move into scene_seen (and are cleared on the device).
+    int sync_read(std::vector<uint32_t>& flags, std::vector<uint32_t>& frozen) {
         HIP_TRY(hipSetDevice(opts.device));
         HIP_TRY(hipStreamSynchronize(stream));
+        const size_t ns = scenes.size();
         uint32_t st = 0;
+        flags.assign(ns, 0u);
+        frozen.assign(ns, 0u);
         HIP_TRY(hipMemcpy(&st, status, sizeof(st), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(flags.data(), scene_flags, ns * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(frozen.data(), scene_frozen, ns * sizeof(uint32_t), hipMemcpyDeviceToHost));
         if (opts.timing) RC_TRY(resolve_events());
-        if (st != 0) {
-            HIP_TRY(nclr_host::launch_fill_u32(status, 1, 0u, stream));
-            HIP_TRY(hipStreamSynchronize(stream));
-            if (st & nclr_dev::kStatusOutOfRange)
-                return fail(MPM_ERR_OUT_OF_RANGE,
-                            "out_of_range: a particle's interpolation stencil left the grid "
-                            "(position not finite or outside [0, 1 - 1.5 dx) on some axis)");
-            if (st & nclr_dev::kStatusMovedTooFar)
-                return fail(MPM_ERR_OUT_OF_RANGE,
-                            "a particle moved more than one tile in one step (the state was not produced "
-                            "by the step; set it with mpm_set_particles)");
-            return fail(MPM_ERR_HIP, "device status " + std::to_string(st));
+        bool any = false;
+        for (size_t s = 0; s < ns; ++s) {
+            scene_seen[s] |= flags[s];
+            any = any || flags[s] != 0u;
         }
+        if (any) HIP_TRY(nclr_host::launch_fill_u32(scene_flags, int64_t(ns), 0u, stream));
+        if (st != 0) HIP_TRY(nclr_host::launch_fill_u32(status, 1, 0u, stream));
+        if (any || st != 0) HIP_TRY(hipStreamSynchronize(stream));
+        if (st & nclr_dev::kStatusMovedTooFar)
+            return fail(MPM_ERR_OUT_OF_RANGE,
+                        "the tile sort placed a particle outside the particle range (the state was not produced "
+                        "by the step; set it with mpm_set_particles)");
+        if (st != 0) return fail(MPM_ERR_HIP, "device status " + std::to_string(st));
+        return MPM_OK;
+    }
+
+    // Engine step (1-based count of steps taken) that ran with device stamp `stamp`, -1 for 0.
+    int64_t step_of_stamp(uint32_t stamp_value) const {
+        return stamp_value ? steps - (stamp_host - int64_t(stamp_value)) : -1;
+    }
+
+    // mpm_synchronize: MPM_ERR_OUT_OF_RANGE naming every scene the steps since the last check
+    // found a particle of off its grid (a stopped scene is found again by every later step).
+    int sync() {
+        std::vector<uint32_t> flags, frozen;
+        RC_TRY(sync_read(flags, frozen));
+        std::string list;
+        int count = 0;
+        uint32_t bits = 0;
+        for (size_t s = 0; s < flags.size(); ++s) {
+            if (!flags[s]) continue;
+            bits |= flags[s];
+            if (++count <= 8)
+                list += (count > 1 ? ", " : "") + std::to_string(s) + " (step " +
+                        std::to_string(step_of_stamp(frozen[s])) + ")";
+        }
+        if (!count) return MPM_OK;
+        if (count > 8) list += ", ... (" + std::to_string(count) + " scenes)";
+        const std::string why = (bits & nclr_dev::kStatusMovedTooFar)
+                                    ? "a particle moved more than one tile in one step (the state was not produced by "
+                                      "the step; set it with mpm_set_particles)"
+                                    : "a particle's interpolation stencil left the grid (position not finite or outside "
+                                      "[0, 1 - 1.5 dx) on some axis)";
+        if (scenes.size() == 1)
+            return fail(MPM_ERR_OUT_OF_RANGE, "out_of_range: " + why + " in step " +
+                                                  std::to_string(step_of_stamp(frozen[0])) +
+                                                  "; the simulation stays at its state before that step");
+        return fail(MPM_ERR_OUT_OF_RANGE, "out_of_range in scene " + list + ": " + why +
+                                              "; those scenes stay at their state before that step, the others "
+                                              "keep stepping");
+    }
+
+    // Clears the per-scene words (the state was replaced: every scene runs again).
+    int clear_scene_state() {
+        HIP_TRY(nclr_host::launch_fill_u32(scene_flags, int64_t(scenes.size()), 0u, stream));
+        HIP_TRY(nclr_host::launch_fill_u32(scene_frozen, int64_t(scenes.size()), 0u, stream));
+        std::fill(scene_seen.begin(), scene_seen.end(), 0u);
+        return MPM_OK;
+    }
+
+    // An adjoint needs every scene live (a stopped scene's step is not the reference's step).
+    int require_live() {
+        std::vector<uint32_t> flags, frozen;
+        RC_TRY(sync_read(flags, frozen));
+        for (size_t s = 0; s < frozen.size(); ++s)
+            if (flags[s] || frozen[s])
+                return fail(MPM_ERR_OUT_OF_RANGE, "scene " + std::to_string(s) +
+                                                      " is stopped (out_of_range); the adjoint needs every scene live: "
+                                                      "set or load a state first");
         return MPM_OK;
     }
 
@@ -324,6 +394,7 @@ struct nclr_mpm_sim {
         }
         cur = 0;
         sorted_valid = false;
+        RC_TRY(clear_scene_state());
         HIP_TRY(hipMemcpyAsync(state[0], host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         HIP_TRY(nclr_host::launch_iota_u32(pid[0], p.stride, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -375,6 +446,7 @@ struct nclr_mpm_sim {
         auto restore = [&]() -> int {
             cur = cur0;
             sorted_valid = false;
+            RC_TRY(clear_scene_state());
             HIP_TRY(hipMemcpyAsync(state[cur], saved, state_bytes, hipMemcpyDeviceToDevice, stream));
             HIP_TRY(hipMemcpyAsync(pid[cur], saved_pid, pid_bytes, hipMemcpyDeviceToDevice, stream));
             HIP_TRY(hipStreamSynchronize(stream));
@@ -407,9 +479,10 @@ struct nclr_mpm_sim {
             }
         if (rc == MPM_OK) rc = restore();
         if (rc == MPM_OK) rc = set_tuning(best_chunk, best_threads);
-        // trial steps may have set the out-of-range flag for states that leave the grid; the real
-        // steps will report it again
+        // trial steps may have stopped scenes whose particles leave the grid; the real steps will
+        // find them again
         if (rc == MPM_OK) rc = nclr_host::launch_fill_u32(status, 1, 0u, stream) == hipSuccess ? MPM_OK : fail(MPM_ERR_HIP, "status reset");
+        if (rc == MPM_OK) rc = clear_scene_state();
         steps = steps0;
         opts.timing = timing0;
         if (e0) pool.push_back(e0);
@@ -436,6 +509,7 @@ struct nclr_mpm_sim {
     int load_state(const void* src) {
         const char* s = static_cast<const char*>(src);
         sorted_valid = false;
+        RC_TRY(clear_scene_state());
         HIP_TRY(hipMemcpyAsync(state[cur], s, state_floats() * sizeof(float), hipMemcpyDeviceToDevice, stream));
         HIP_TRY(hipMemcpyAsync(pid[cur], s + state_floats() * sizeof(float), size_t(p.stride) * sizeof(uint32_t),
                                hipMemcpyDeviceToDevice, stream));
@@ -501,16 +575,22 @@ struct nclr_mpm_sim {
         const nclr_host::AdjointArgs b{gout, gin, p.n, slots_b, gbar, epart, grad_E, erange, epartial, echunks};
         PendingEvent pe{};
         if (opts.timing) RC_TRY(record_start(pe, kBackward));
-        if (presorted)
+        if (presorted) {
             HIP_TRY(nclr_host::launch_bump_u32(stamp, stream));
-        else
+        } else {
+            // the radix sort rewrites the sort block (keys, tile bounds, work items) for the current
+            // state, whose slots are still in the previous step's order: the next step must sort
+            // from scratch, not incrementally from that block
             RC_TRY(timed(kSort, [&] { return nclr_host::launch_sort(a); }));
+            sorted_valid = false;
+        }
         RC_TRY(timed(kP2G, [&] { return nclr_host::launch_p2g(a); }));
         RC_TRY(timed(kGrid, [&] { return nclr_host::launch_grid(a); }));
         RC_TRY(timed(kG2PAdj, [&] { return nclr_host::launch_g2p_adjoint(a, b); }));
         RC_TRY(timed(kGridAdj, [&] { return nclr_host::launch_grid_adjoint(a, b); }));
         RC_TRY(timed(kP2GAdj, [&] { return nclr_host::launch_p2g_adjoint(a, b); }));
         if (opts.timing) RC_TRY(record_stop(pe));
+        stamp_host += 1;
         return MPM_OK;
     }
 
@@ -618,13 +698,15 @@ struct nclr_mpm_sim {
                         hist,     slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
                         gtmp[1],  tape,       sort_tape, uidx,     ucnt,        upref, ttotal, rtot,
-                        tile_scene, node_tile_scene};
+                        tile_scene, node_tile_scene, scene_flags, scene_frozen};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         uidx = ucnt = upref = ttotal = meta = rtot = nullptr;
         p.items = nullptr;
         tile_scene = node_tile_scene = nullptr;
+        scene_flags = scene_frozen = nullptr;
         p.tile_scene = p.node_tile_scene = nullptr;
+        p.scene_flags = p.scene_frozen = nullptr;
         slots_b = gbar = nullptr;
         epart = gtmp[0] = gtmp[1] = nullptr;
         erange = nullptr;
@@ -751,10 +833,14 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
         dmalloc(&h->d_scenes, size_t(nscenes)) != hipSuccess || dmalloc(&h->uidx, nn) != hipSuccess ||
         dmalloc(&h->ucnt, bucket_cells) != hipSuccess || dmalloc(&h->upref, bucket_cells) != hipSuccess ||
         dmalloc(&h->ttotal, nbins) != hipSuccess ||
-        dmalloc(&h->rtot, 2 * (size_t(tiles) / nclr_dev::kScanRound + 2)) != hipSuccess)
+        dmalloc(&h->rtot, 2 * (size_t(tiles) / nclr_dev::kScanRound + 2)) != hipSuccess ||
+        dmalloc(&h->scene_flags, size_t(nscenes)) != hipSuccess || dmalloc(&h->scene_frozen, size_t(nscenes)) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "hipMalloc failed (out of device memory?)"));
     p.scenes = h->d_scenes;
     p.stamp = h->stamp;
+    p.scene_flags = h->scene_flags;
+    p.scene_frozen = h->scene_frozen;
+    h->scene_seen.assign(size_t(nscenes), 0u);
     if (nscenes > 1) {
         std::vector<uint32_t> ts(static_cast<size_t>(tiles)), nts(static_cast<size_t>(node_tiles));
         for (int s = 0; s < nscenes; ++s) {
@@ -843,6 +929,17 @@ int mpm_advance(mpm_handle h, int nsteps) {
 int mpm_synchronize(mpm_handle h) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
     return h->sync();
+}
+
+int mpm_scene_status(mpm_handle h, uint32_t* flags, int64_t* failed_step) {
+    if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
+    std::vector<uint32_t> f, frozen;
+    RC_TRY(h->sync_read(f, frozen));
+    for (size_t s = 0; s < frozen.size(); ++s) {
+        if (flags) flags[s] = h->scene_seen[s];
+        if (failed_step) failed_step[s] = h->step_of_stamp(frozen[s]);
+    }
+    return MPM_OK;
 }
 
 int mpm_get_particles(mpm_handle h, float* x, float* v, float* F, float* C, float* Jp, float* mass,
@@ -1099,7 +1196,7 @@ int mpm_set_state_device(mpm_handle h, const float* src) {
     HIP_TRY(nclr_host::launch_iota_u32(h->pid[o], stride, h->stream));
     h->cur = o;
     h->sorted_valid = false;
-    return MPM_OK;
+    return h->clear_scene_state();
 }
 
 int mpm_grad_from_particle_order(mpm_handle h, const float* src, float* dst) {
@@ -1124,13 +1221,20 @@ int mpm_backward_step(mpm_handle h, const float* grad_out, float* grad_in, doubl
     if (!h || ((!grad_out || !grad_in) && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
     if (grad_out == grad_in && h->p.n > 0) return fail(MPM_ERR_INVALID, "grad_out and grad_in must not alias");
     HIP_TRY(hipSetDevice(h->opts.device));
+    RC_TRY(h->require_live());
     return h->backward_step(grad_out, grad_in, grad_E);
 }
 
 int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float* grad_in, double* grad_E) {
     if (!h || ((!grad_out || !grad_in) && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
     HIP_TRY(hipSetDevice(h->opts.device));
+    RC_TRY(h->require_live());
     return h->backward_segment(nsteps, grad_out, grad_in, grad_E);
+}
+
+int64_t mpm_tape_bytes(mpm_handle h, int nsteps) {
+    if (!h || nsteps < 1) return -1;
+    return int64_t(nsteps - 1) * int64_t(h->state_bytes() + h->sort_block_bytes);
 }
 
 int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu) {

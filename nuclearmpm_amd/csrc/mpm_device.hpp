@@ -52,6 +52,11 @@ NCLR_DI int scene_of_sorted(const StepParams& p, const uint32_t* __restrict__ ke
     return key < p.ntiles ? int(p.tile_scene[key]) : scene_of_particle(p, id);
 }
 
+// A stopped scene (src/nclr.h:163-164: std::vector::at throws out of that scene's advance(), which
+// leaves its particles as they were before the step) takes no further part in a step.
+NCLR_DI bool scene_stopped(const StepParams& p, int s) { return (p.scene_flags[s] | p.scene_frozen[s]) != 0u; }
+NCLR_DI void stop_scene(const StepParams& p, int s, uint32_t bits) { atomicOr(&p.scene_flags[s], bits); }
+
 // base = trunc(x*inv_dx - 0.5), fx = x*inv_dx - base (src/nclr.h:115,121), with the stencil
 // nodes base..base+2 required to lie in [0, res] on every axis.
 template <int D>
@@ -312,31 +317,34 @@ NCLR_DI void round_carries(const uint32_t* __restrict__ rtot, uint32_t round, ui
     block_scan2(a, b, ea, eb, cp, ci);
 }
 
-// Exact LDS reductions in fixed point.  A workgroup bounds the sum of |contributions| of every
-// channel (`total`) and scales the channel by 2^e with total * 2^e <= 2^61: each quantised
-// contribution is below 2^62 in magnitude and the 64-bit integer sum is exact.  The quantum,
-// 2^-61 of the bound, stays below the FP32 rounding of even the small sums of a region's edge
-// nodes (a coarser quantum, 2^-50, measurably moves ill-conditioned 3D snow particles).
-constexpr float kFixedLimit = 4.6116860e18f;  // 2^62
+// Exact LDS reductions in fixed point.  A workgroup bounds every particle's |contributions| per
+// channel (each single contribution is at most that particle's bound) and scales the channel by
+// 2^e with (largest particle bound) * 2^e < 2^50: every quantised contribution is below 2^50 in
+// magnitude, so a node's sum over the at most 1024 particles of a work item stays below 2^60 and
+// the 64-bit integer sum is exact and independent of the order of the atomics.  The quantum, 2^-50
+// of the largest particle bound, is below the FP32 rounding of every node sum above 1e-8 of that
+// bound.  A particle whose bound is not finite is deposited with a check per contribution:
+// contributions at or beyond kFixedLimit flag their node channel (NaN) instead.
+constexpr float kFixedLimit = 1.12589991e15f;  // 2^50
 
-NCLR_DI void fixed_point_scale(float total, float& scale, double& inv_scale) {
+NCLR_DI void fixed_point_scale(float bound_max, float& scale, double& inv_scale) {
     int ex = 0;
-    (void)frexpf(total > 0.0f && total < 3.0e38f ? total : 1.0f, &ex);  // total < 2^ex
-    const int e = min(127, 61 - ex);                                      // in [-68, 127]
+    (void)frexpf(bound_max > 0.0f && bound_max < 3.0e38f ? bound_max : 1.0f, &ex);  // bound_max < 2^ex
+    const int e = min(127, 50 - ex);                                                  // in [-78, 127]
     scale = ldexpf(1.0f, e);
     inv_scale = ldexp(1.0, -e);
 }
 
-// y (|y| < 2^62) quantised to a two's complement 64-bit value, in double: hi = floor(y 2^-32) is
-// the high word and lo = y - hi 2^32 in [0, 2^32], truncated by a saturating conversion, the low
-// word.  That is floor(y) for every such float except four in (-4, -1] (-(1 + 2^-23), -(1 + 2^-22),
-// -(2 + 2^-22), -(3 + 2^-22)), where 2^32 + y rounds up to an integer and the result is floor(y) + 1:
-// still deterministic and within one unit, which is all the exact sums need.
+// y (|y| < 2^51) rounded to the nearest integer (ties to even), as a two's complement 64-bit
+// value: in double, y + 1.5 * 2^52 lies in [2^52, 2^53), where the unit in the last place is 1, so
+// its significand holds round(y) + 2^51 and its bit pattern minus that of 1.5 * 2^52 is round(y).
+// The constant's low word is zero: only the high word is adjusted.  Three instructions
+// (convert, add, subtract); deterministic, which is all the exact sums need.
 NCLR_DI unsigned long long fixed_point(float y) {
-    const double r = double(y);
-    const double hi = floor(r * 2.3283064365386963e-10);  // 2^-32
-    const double lo = fma(hi, -4294967296.0, r);
-    return (unsigned long long)(uint32_t)lo | ((unsigned long long)(uint32_t)(int32_t)hi << 32);
+    const double t = double(y) + 6755399441055744.0;  // 1.5 * 2^52
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(t));
+    const uint32_t lo = uint32_t(b), hi = uint32_t(b >> 32) - 0x43380000u;
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
 NCLR_DI float fixed_point_value(unsigned long long q, double inv_scale) {

@@ -71,6 +71,12 @@ struct StepParams {
                             // (written with item_offset)
     const uint32_t* tile_scene;       // ntiles: scene of each tile (batches of more than one scene)
     const uint32_t* node_tile_scene;  // node_tiles: scene of each node tile (likewise)
+    // Per scene: status bits the sort raised since the host last read them (a particle off its
+    // grid), and the device step stamp of the step that found the scene's first failure (0: live).
+    // A scene with either word set is stopped: its particles are carried through unchanged and
+    // its grid is not touched, while the other scenes keep stepping.
+    uint32_t* scene_flags;
+    uint32_t* scene_frozen;
 };
 
 }  // namespace nclr_dev

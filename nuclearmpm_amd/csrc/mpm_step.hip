@@ -49,7 +49,8 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ st, cons
         tile_end[i] = 0;
     }
     if (i >= p.n) return;
-    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_particle(p, pid[i])];
+    const int s = p.nscenes == 1 ? 0 : scene_of_particle(p, pid[i]);
+    const SceneParams sp = p.scenes[s];
     float x[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) x[a] = ldf(st, p.stride, Layout<D>::X + a, i);
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ st, cons
         keys[i] = sp.tile_base + tile_key<D>(base, sp.nt);
     } else {
         keys[i] = p.ntiles;
-        atomicOr(status, kStatusOutOfRange);
+        stop_scene(p, s, kStatusOutOfRange);
     }
 }
 
@@ -222,11 +223,13 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
     const uint32_t item = blockIdx.x;
     const WorkItem wi = work_item(p, item);  // loaded with the count below, used only if valid
     if (item >= item_offset[p.ntiles]) return;
+    const uint32_t tile = wi.tile;
+    const int scene = p.nscenes == 1 ? 0 : scene_of_tile(p, tile);
+    if (scene_stopped(p, scene)) return;  // uniform over the workgroup
     for (int r = threadIdx.x; r < 4 * REG; r += blockDim.x) acc[r] = 0ull;
     for (int r = threadIdx.x; r < REG; r += blockDim.x) nonfinite[r] = 0u;
     __syncthreads();
-    const uint32_t tile = wi.tile;
-    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_tile(p, tile)];
+    const SceneParams sp = p.scenes[scene];
     const int material = MAT == kMixed ? sp.material : MAT;
     int torg[D];
     tile_coords<D>(tile - sp.tile_base, sp.nt, torg);
@@ -288,18 +291,18 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
             for (int b = 0; b < D; ++b) s += fabsf(As[k].a[c][b]);
             const float bc = wsum * (fabsf(mvs[k][c]) + 2.5f * sp.dx * s);
             finite[k] = finite[k] && bc < 3.0e38f;
-            if (bc < 3.0e38f) bound[c] += bc;  // non-finite contributions take the flag path
+            if (bc < 3.0e38f) bound[c] = fmaxf(bound[c], bc);  // non-finite contributions take the flag path
         }
         const float bm = wsum * fabsf(mass);
         finite[k] = finite[k] && bm < 3.0e38f;
-        if (bm < 3.0e38f) bound[3] += bm;
+        if (bm < 3.0e38f) bound[3] = fmaxf(bound[3], bm);
     }
-    // workgroup sum of the bounds -> per channel scale 2^e with bound * 2^e <= 2^50
+    // workgroup maximum of the particle bounds -> per channel scale 2^e with bound * 2^e < 2^50
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         float b = bound[c];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off);
+        for (int off = 32; off > 0; off >>= 1) b = fmaxf(b, __shfl_xor(b, off));
         bound[c] = b;
     }
     if ((threadIdx.x & 63) == 0)
@@ -310,12 +313,12 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
     double inv_scale[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float total = (s_bound[0][c] + s_bound[1][c]) + (s_bound[2][c] + s_bound[3][c]);
-        fixed_point_scale(total, scale[c], inv_scale[c]);
+        const float bmax = fmaxf(fmaxf(s_bound[0][c], s_bound[1][c]), fmaxf(s_bound[2][c], s_bound[3][c]));
+        fixed_point_scale(bmax, scale[c], inv_scale[c]);
     }
 
     // pass 2: quantised contributions -> 64-bit integer LDS atomics.  A particle whose bound is
-    // finite has every |contribution * scale| below 2^51 and needs no per-contribution check.
+    // finite has every |contribution * scale| below 2^50 and needs no per-contribution check.
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
@@ -353,7 +356,9 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
     __shared__ uint32_t s_active;
     const uint32_t nt_idx = xcd_block();
     if (nt_idx >= p.node_tiles) return;
-    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_node_tile(p, nt_idx)];
+    const int scene = p.nscenes == 1 ? 0 : scene_of_node_tile(p, nt_idx);
+    if (scene_stopped(p, scene)) return;  // a stopped scene's grid keeps its last completed step
+    const SceneParams sp = p.scenes[scene];
     int nto[D];
     tile_coords<D>(nt_idx - sp.node_tile_base, sp.ntn, nto);
     bool occupied = false;
@@ -503,7 +508,8 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
     const int64_t s = p.stride;
     const uint32_t id = pid_in[i];
     pid_out[j] = id;
-    const SceneParams sp = p.scenes[p.nscenes == 1 ? 0 : scene_of_sorted(p, keys, j, id)];
+    const int scene = p.nscenes == 1 ? 0 : scene_of_sorted(p, keys, j, id);
+    const SceneParams sp = p.scenes[scene];
     const int material = MAT == kMixed ? sp.material : MAT;
     float x[D];
     M<D> F;
@@ -518,8 +524,11 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
     out[int64_t(L::VOL) * s + j] = ldf(st, s, L::VOL, i);
     int base[D];
     float fx[D], w[D][3];
-    if (!stencil<D>(x, sp, base, fx, w)) {
-        // outside the grid (already reported by the key kernel): carried through unchanged
+    if (scene_stopped(p, scene) || !stencil<D>(x, sp, base, fx, w)) {
+        // a stopped scene (the sort found one of its particles off the grid in this or an earlier
+        // step): carried through unchanged; the step that stopped it is recorded (same value from
+        // every particle of the scene)
+        if (p.scene_frozen[scene] == 0u) p.scene_frozen[scene] = *p.stamp;
 #pragma unroll
         for (int f = 0; f < L::JP + 1; ++f) out[int64_t(f) * s + j] = ldf(st, s, f, i);
         return;
@@ -616,9 +625,11 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
 
 // Dense export of one scene's updated grid in the reference's flat x-major order
 // (src/nclr.h:87); nodes of node tiles the last step did not touch are empty cells (zero).
+// A stopped scene shows the grid of its last completed step.
 template <int D>
 __global__ __launch_bounds__(256) void k_export_grid(const float4* __restrict__ gv, const uint32_t* __restrict__ stamps,
-                                                     const uint32_t* __restrict__ stamp, int res,
+                                                     const uint32_t* __restrict__ stamp,
+                                                     const uint32_t* __restrict__ frozen, int res,
                                                      uint32_t node_tile_base, int ntn, float* __restrict__ vel,
                                                      float* __restrict__ mass) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -634,7 +645,7 @@ __global__ __launch_bounds__(256) void k_export_grid(const float4* __restrict__ 
         r /= side;
     }
     const int64_t node = gv_index<D>(n, ntn, node_tile_base);
-    const uint32_t now = *stamp;
+    const uint32_t now = *frozen != 0u ? *frozen - 1u : *stamp;
     const bool live = now != 0 && stamps[node / Tile<D>::NODES] == now;
     const float4 g = live ? gv[node] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     vel[idx * D + 0] = g.x;
@@ -808,13 +819,13 @@ hipError_t launch_active_sum(const LaunchArgs& a, uint32_t first_tile, uint32_t 
 
 hipError_t launch_export_grid(const LaunchArgs& a, int scene, int res, uint32_t node_tile_base, int ntn, float* vel,
                               float* mass, int64_t total) {
-    (void)scene;
+    const uint32_t* frozen = a.p.scene_frozen + scene;
     if (a.dim == 2)
-        k_export_grid<2><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.stamp, res, node_tile_base,
-                                                                       ntn, vel, mass);
+        k_export_grid<2><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.stamp, frozen, res,
+                                                                       node_tile_base, ntn, vel, mass);
     else
-        k_export_grid<3><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.stamp, res, node_tile_base,
-                                                                       ntn, vel, mass);
+        k_export_grid<3><<<blocks_for(total, 256), 256, 0, a.stream>>>(a.gv, a.stamps, a.stamp, frozen, res,
+                                                                       node_tile_base, ntn, vel, mass);
     return hipGetLastError();
 }
 

@@ -16,7 +16,9 @@ Differences from the reference, all deliberate:
   * advance(steps) takes an optional step count (queued on the device without waiting);
   * a stencil leaving the grid raises IndexError (the reference's std::out_of_range) at the next
     call that reads results, and is raised for every axis, where the reference aliases silently
-    into the neighbouring row on the faster axes.
+    into the neighbouring row on the faster axes.  The scene it happens in stops at its state
+    before that step (what the reference's throwing advance() leaves); in a batch the other scenes
+    keep stepping, and scene_status() lists the stopped scenes without raising.
 """
 from __future__ import annotations
 
@@ -226,6 +228,17 @@ class _Engine:
 
     def synchronize(self) -> None:
         check(lib().mpm_synchronize(self._handle()))
+
+    def scene_status(self) -> dict:
+        """Stopped scenes without raising: {scene: {"flags": status bits, "step": the step that
+        stopped it}} (bit 1: a particle's stencil left the grid, the reference's std::out_of_range;
+        bit 2: a particle crossed more than one tile).  Waits for queued steps.  A scene stops on its
+        own (its particles stay as they were before that step) while the others keep stepping."""
+        ns = self.scenes
+        flags = np.zeros(ns, np.uint32)
+        steps = np.zeros(ns, np.int64)
+        check(lib().mpm_scene_status(self._handle(), ptr(flags), ptr(steps)))
+        return {s: {"flags": int(flags[s]), "step": int(steps[s])} for s in range(ns) if flags[s] or steps[s] >= 0}
 
     def kernel_time(self, name: str):
         """(average ms per launch, launches) of kernel group `name` ("sort", "p2g", "grid", "g2p",

@@ -396,25 +396,44 @@ class Simulation {
 
     // One particle's 3^D contributions (nclr.h:110-158) added into `grid`.
     void deposit(const Particle<T, D>& p, std::vector<Cell<T, D>>& grid) const {
+        Prologue q;
+        prologue(p, q);
+        scatter(q, grid, std::numeric_limits<int>::min(), std::numeric_limits<int>::max());
+    }
+
+    // What a particle's deposit needs besides the grid: its stencil and fused stress (nclr.h:112-137).
+    struct Prologue {
         int base[D];
         T fx[D], w[D][3];
-        base_and_weights(p.x, base, fx, w);
         T affine[D][D];
-        first_piola_kirchoff_stress(p, affine);
         T mv[D];
-        for (int a = 0; a < D; ++a) mv[a] = p.v[a] * p.mass;
+        T mass;
+    };
+
+    void prologue(const Particle<T, D>& p, Prologue& q) const {
+        base_and_weights(p.x, q.base, q.fx, q.w);
+        first_piola_kirchoff_stress(p, q.affine);
+        for (int a = 0; a < D; ++a) q.mv[a] = p.v[a] * p.mass;
+        q.mass = p.mass;
+    }
+
+    // The contributions of one prologue to the nodes whose first index lies in [row_lo, row_hi)
+    // (nclr.h:138-165), in the reference's offset order.
+    void scatter(const Prologue& q, std::vector<Cell<T, D>>& grid, int row_lo, int row_hi) const {
         for_each_offset([&](const int off[D]) {
+            const int row = q.base[0] + off[0];
+            if (row < row_lo || row >= row_hi) return;
             T dpos[D];
-            for (int a = 0; a < D; ++a) dpos[a] = (T(off[a]) - fx[a]) * dx_;
-            T weight = w[0][off[0]];
-            for (int a = 1; a < D; ++a) weight = weight * w[a][off[a]];
-            Cell<T, D>& cell = cell_in(grid, base, off);
+            for (int a = 0; a < D; ++a) dpos[a] = (T(off[a]) - q.fx[a]) * dx_;
+            T weight = q.w[0][off[0]];
+            for (int a = 1; a < D; ++a) weight = weight * q.w[a][off[a]];
+            Cell<T, D>& cell = cell_in(grid, q.base, off);
             for (int r = 0; r < D; ++r) {
-                T ad = affine[r][0] * dpos[0];
-                for (int c = 1; c < D; ++c) ad = ad + affine[r][c] * dpos[c];
-                cell.v[r] = cell.v[r] + weight * (mv[r] + ad);
+                T ad = q.affine[r][0] * dpos[0];
+                for (int c = 1; c < D; ++c) ad = ad + q.affine[r][c] * dpos[c];
+                cell.v[r] = cell.v[r] + weight * (q.mv[r] + ad);
             }
-            cell.m = cell.m + weight * p.mass;
+            cell.m = cell.m + weight * q.mass;
         });
     }
 
@@ -446,53 +465,51 @@ class Simulation {
         }
     }
 
-    // The same step on `threads` OpenMP threads (the multi-core CPU baseline; the reference itself
-    // is serial): P2G into one private grid per thread over a static split of the particles, the
-    // private grids summed per node in thread order, then the grid update and G2P split over nodes
-    // and particles.  Only P2G's summation order differs from advance().
+    // The same step on `threads` OpenMP threads, bitwise equal to advance(): the per-particle
+    // prologues (stencil and stress, the costly part) in parallel; then each thread owns a band of
+    // first-axis node rows and adds, in particle order, the contributions that land in its band,
+    // so every node receives its contributions in the serial order; then the grid update over
+    // nodes and G2P over particles, both per element.  The first and last bands are unbounded, so
+    // an out-of-range node still throws (std::vector::at in the reference).
     void advance_parallel(int threads) {
+        threads = std::max(threads, 1);
         const int64_t nn = num_nodes();
         const int64_t np = int64_t(particles_.size());
-        std::vector<std::vector<Cell<T, D>>> priv(size_t(std::max(threads, 1)));
+        std::vector<Prologue>& pro = prologues_;  // kept between steps (no reallocation)
+        pro.resize(size_t(np));
         std::exception_ptr err = nullptr;
-        cells_.resize(size_t(nn));
+        auto guard = [&](auto&& fn) {
+            try {
+                fn();
+            } catch (...) {
+#pragma omp critical
+                if (!err) err = std::current_exception();
+            }
+        };
+        zero_grid(cells_);
 #pragma omp parallel num_threads(threads)
         {
+#pragma omp for schedule(static)
+            for (int64_t pp = 0; pp < np; ++pp) guard([&] { prologue(particles_[size_t(pp)], pro[size_t(pp)]); });
             int t = 0, nt = 1;
 #ifdef _OPENMP
             t = omp_get_thread_num();
             nt = omp_get_num_threads();
 #endif
-            try {
-                auto& g = priv[size_t(t)];
-                zero_grid(g);
-                const int64_t a = np * t / nt, b = np * (t + 1) / nt;
-                for (int64_t pp = a; pp < b; ++pp) deposit(particles_[size_t(pp)], g);
-            } catch (...) {
-#pragma omp critical
-                if (!err) err = std::current_exception();
-            }
+            const int rows = res_ + 1;
+            const int lo = t == 0 ? std::numeric_limits<int>::min() : int(int64_t(rows) * t / nt);
+            const int hi = t == nt - 1 ? std::numeric_limits<int>::max() : int(int64_t(rows) * (t + 1) / nt);
+            guard([&] {
+                for (int64_t pp = 0; pp < np; ++pp) {
+                    const Prologue& q = pro[size_t(pp)];
+                    if (q.base[0] + 2 >= lo && q.base[0] < hi) scatter(q, cells_, lo, hi);
+                }
+            });
 #pragma omp barrier
 #pragma omp for schedule(static)
-            for (int64_t i = 0; i < nn; ++i) {
-                Cell<T, D> c = priv[0][size_t(i)];
-                for (int k = 1; k < nt; ++k) {
-                    const Cell<T, D>& o = priv[size_t(k)][size_t(i)];
-                    for (int a = 0; a < D; ++a) c.v[a] = c.v[a] + o.v[a];
-                    c.m = c.m + o.m;
-                }
-                cells_[size_t(i)] = c;
-                update_node(i);
-            }
+            for (int64_t i = 0; i < nn; ++i) update_node(i);
 #pragma omp for schedule(static)
-            for (int64_t pp = 0; pp < np; ++pp) {
-                try {
-                    update_particle(particles_[size_t(pp)]);
-                } catch (...) {
-#pragma omp critical
-                    if (!err) err = std::current_exception();
-                }
-            }
+            for (int64_t pp = 0; pp < np; ++pp) guard([&] { update_particle(particles_[size_t(pp)]); });
         }
         if (err) std::rethrow_exception(err);
     }
@@ -616,6 +633,7 @@ class Simulation {
     }
 
   private:
+    std::vector<Prologue> prologues_;  // advance_parallel's per-particle prologues
     template <typename Fn>
     void for_each_offset(Fn&& fn) const {
         int off[D];

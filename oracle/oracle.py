@@ -222,13 +222,16 @@ def time_steps(state: State, material: int, res: int, dt: float = 1e-4, E: float
 
 def run_threads(state: State, material: int, res: int = 64, dt: float = 1e-4, E: float = 1e4, nu: float = 0.2,
                 gravity: float = -100.0, steps: int = 1, identity_mode: int = 0, threads: int = 1,
-                fast: bool = True) -> State:
-    """FP32 steps of the oracle on `threads` OpenMP threads (the fast build has OpenMP)."""
-    s = state.astype(np.float32)
-    lib = _lib(fast=fast)
-    rc = lib.oracle_run_threads_f32(
-        ctypes.c_int(s.dim), ctypes.c_int(material), ctypes.c_int(res), ctypes.c_float(dt), ctypes.c_float(E),
-        ctypes.c_float(nu), ctypes.c_float(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(s.n),
+                fast: bool = True, fp64: bool = False) -> State:
+    """Steps of the oracle on `threads` OpenMP threads: FP32 (by default on the fast build, the CPU
+    baseline) or FP64 (always on the exact build).  The threaded step is bitwise the serial one:
+    every node receives its contributions in particle order."""
+    dt_, cty, suf = _ty(fp64)
+    s = state.astype(dt_)
+    lib = _lib(fast=fast and not fp64)
+    rc = getattr(lib, f"oracle_run_threads_{suf}")(
+        ctypes.c_int(s.dim), ctypes.c_int(material), ctypes.c_int(res), cty(dt), cty(E),
+        cty(nu), cty(gravity), ctypes.c_int(identity_mode), ctypes.c_int64(s.n),
         ctypes.c_int(steps), ctypes.c_int(threads), _ptr(s.x), _ptr(s.v), _ptr(s.F), _ptr(s.C), _ptr(s.Jp),
         _ptr(s.mass), _ptr(s.volume))
     _check(rc, lib)

@@ -321,6 +321,17 @@ int oracle_run_threads_f32(int dim, int material, int res, float dt, float E, fl
                                        nullptr, nullptr, threads)));
 }
 
+// FP64 steps on `threads` OpenMP threads (bitwise the serial step).
+int oracle_run_threads_f64(int dim, int material, int res, double dt, double E, double nu, double g, int idm,
+                           int64_t n, int nsteps, int threads, double* x, double* v, double* F, double* C, double* Jp,
+                           const double* mass, const double* volume) {
+    return DIM_DISPATCH(dim,
+                        (run<double, 2>(material, res, dt, E, nu, g, idm, n, nsteps, x, v, F, C, Jp, mass, volume,
+                                        nullptr, nullptr, threads)),
+                        (run<double, 3>(material, res, dt, E, nu, g, idm, n, nsteps, x, v, F, C, Jp, mass, volume,
+                                        nullptr, nullptr, threads)));
+}
+
 double oracle_time_steps_f32(int dim, int material, int res, float dt, float E, float nu, float g, int idm,
                              int64_t n, int warmup, int steps, const float* x, const float* v, const float* F,
                              const float* C, const float* Jp, const float* mass, const float* volume, int threads) {

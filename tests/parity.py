@@ -2,22 +2,28 @@
 
 One-step agreement is measured per field as a normwise relative error
     err(f) = max|f_gpu - f_ref| / max|f_ref|
-for x, v, F and Jp.  The APIC matrix C is a weighted difference of grid velocities,
-C = 4 inv_dx sum_i w_i v_i (i - fx)^T, whose FP32 rounding error scales with
-4 inv_dx max|v| rather than with max|C|: during free fall |v| >> |C| and two correct FP32
-evaluations that only differ in the P2G summation order disagree by up to 5e-5 of max|C|
-(measured with the oracle against itself on a permuted particle order).  C is therefore
-normalised by max(max|C_ref|, 4 inv_dx max|v_ref|).
+for x, v, F and Jp, each bounded by TOL = 1e-5.
 
-Alongside, `<field>_elem` reports the elementwise relative error max |f_gpu - f_ref| / |f_ref| over
-the elements with |f_ref| > 1e-3 max|f_ref| (reported, not bounded: small elements of a field carry
-the rounding of its large ones).
+C is bounded by TOL on
+    max|C_gpu - C_ref| / max(max|C_ref|, C_FLOOR * 4 inv_dx max|v_ref|)
+with C_FLOOR = 0.1: the plain normwise error wherever C is at least a tenth of its velocity scale
+4 inv_dx max|v|, as in every mid-trajectory state.  C is a weighted difference of grid velocities,
+C = 4 inv_dx sum_i w_i v_i (i - fx)^T, so its FP32 rounding scales with 4 inv_dx max|v|, not with
+max|C|: in a cube falling at rest (C is rounding noise, max|C| ~ 1e-7 of that scale) no two FP32
+evaluations agree on C to 1e-5 of max|C|, and two runs of the reference's own step that differ
+only in P2G's summation order disagree by up to 5e-5 of max|C| in free fall.
+
+Reported beside them, not bounded:
+  * `C_plain`: max|dC| / max|C_ref|, and `C_scaled`: max|dC| / max(max|C_ref|, 4 inv_dx max|v_ref|);
+  * `<field>_elem`: the elementwise relative error max |f_gpu - f_ref| / |f_ref| over the elements
+    with |f_ref| > 1e-3 max|f_ref| (small elements of a field carry the rounding of its large ones).
 """
 from __future__ import annotations
 
 import numpy as np
 
 TOL = 1e-5  # one-step relative agreement required for x, v, C, F (and Jp)
+C_FLOOR = 0.1  # C is normalised by max(max|C|, C_FLOOR * 4 inv_dx max|v|)
 ELEM_FLOOR = 1e-3  # elementwise errors are taken over |f_ref| > ELEM_FLOOR * max|f_ref|
 
 
@@ -43,21 +49,28 @@ def field_errors(gpu, ref, inv_dx: float) -> dict:
     b = np.asarray(ref.C, np.float64)
     vmax = np.abs(np.asarray(ref.v, np.float64)).max(initial=0.0)
     den = max(np.abs(b).max(initial=0.0), 4.0 * inv_dx * vmax)
-    out["C"] = float(np.abs(a - b).max(initial=0.0) / den) if den > 0 else float(np.abs(a - b).max(initial=0.0))
-    out["C_plain"] = float(np.abs(a - b).max(initial=0.0) / max(np.abs(b).max(initial=0.0), 1e-30))
+    dC = np.abs(a - b).max(initial=0.0)
+    cmax = np.abs(b).max(initial=0.0)
+    cden = max(cmax, C_FLOOR * 4.0 * inv_dx * vmax)
+    out["C"] = float(dC / cden) if cden > 0 else float(dC)
+    out["C_plain"] = float(dC / cmax) if cmax > 0 else float(dC)
+    out["C_scaled"] = float(dC / den) if den > 0 else float(dC)
     for k in ("x", "v", "F", "C", "Jp"):
         out[k + "_elem"] = elementwise_error(getattr(gpu, k), getattr(ref, k))
     return out
 
 
-def bounded(errs: dict) -> dict:
-    """The normwise errors the parity bound applies to."""
-    return {k: v for k, v in errs.items() if k != "C_plain" and not k.endswith("_elem")}
+def bounded(errs: dict, scaled_C: bool = False) -> dict:
+    """The normwise errors the parity bound applies to (C on the plain normalisation, or on the
+    scaled one for two FP32 float-summing paths compared with each other)."""
+    out = {k: v for k, v in errs.items() if k not in ("C", "C_plain", "C_scaled") and not k.endswith("_elem")}
+    out["C"] = errs["C_scaled"] if scaled_C else errs["C"]
+    return out
 
 
-def assert_parity(gpu, ref, inv_dx: float, tol: float = TOL, label: str = "") -> dict:
+def assert_parity(gpu, ref, inv_dx: float, tol: float = TOL, label: str = "", scaled_C: bool = False) -> dict:
     errs = field_errors(gpu, ref, inv_dx)
-    bad = {k: v for k, v in bounded(errs).items() if not (v <= tol)}
+    bad = {k: v for k, v in bounded(errs, scaled_C).items() if not (v <= tol)}
     assert not bad, f"{label}: parity above {tol}: {bad} (all: {errs})"
     return errs
 

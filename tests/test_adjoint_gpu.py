@@ -112,6 +112,60 @@ def test_backward_step_c_abi_and_permutations():
     assert abs(float(gE) - ref.E) <= 1e-4 * abs(ref.E)
 
 
+def test_backward_step_leaves_the_following_steps_unchanged():
+    """mpm_backward_step sorts the current state itself; the next advance must then sort from
+    scratch, so stepping on after it equals stepping on without it, bit for bit."""
+    d = 2
+    p = jelly_particles(d, 20, seed=9)
+    n, ng = p.n, autodiff.channels(d)
+    rng = np.random.default_rng(4)
+    with MPMSimulation(p, MaterialModel.kJelly, res=64) as a, MPMSimulation(p, MaterialModel.kJelly, res=64) as b:
+        a.advance(5)
+        b.advance(5)
+        L, h = lib(), a._handle()
+        gout = torch.from_numpy(rng.normal(size=(ng, n)).astype(np.float32)).cuda()
+        gin = torch.empty_like(gout)
+        gE = torch.zeros(1, dtype=torch.float64, device="cuda")
+        check(L.mpm_backward_step(h, gout.data_ptr(), gin.data_ptr(), gE.data_ptr()))
+        check(L.mpm_synchronize(h))
+        for k in range(3):
+            a.advance(4)
+            b.advance(4)
+            pa, pb = a.particles(), b.particles()
+            for f in FIELDS + ("Jp",):
+                np.testing.assert_array_equal(getattr(pa, f), getattr(pb, f), err_msg=f"round {k} field {f}")
+
+
+def test_two_rollouts_with_different_E_then_one_backward():
+    """Two rollouts of one simulation at different Young's moduli, differentiated by one backward
+    pass: each rollout's adjoint recomputes its segments with its own E, so the gradients equal
+    those of the two rollouts run and differentiated separately."""
+    d, steps, every = 2, 16, 8
+    p = affine_jelly(d, 16, 0.35, 0.55, seed=12)
+    n = p.n
+    rng = np.random.default_rng(8)
+    w = [autodiff.pack(*(torch.from_numpy(rng.normal(size=getattr(p, k).shape).astype(np.float32)).cuda()
+                         for k in FIELDS), Jp=torch.zeros(n, device="cuda")) for _ in range(2)]
+    Es = (3e3, 8e3)
+    with MPMSimulation(p, MaterialModel.kJelly, res=64, E=5e3) as sim:
+        s0 = to_state(p).requires_grad_(True)
+        E = [torch.tensor([e], dtype=torch.float64, requires_grad=True) for e in Es]
+        outs = [autodiff.rollout(sim, s0, E[i], steps, every) for i in range(2)]
+        sum((outs[i] * w[i]).sum() for i in range(2)).backward()
+        g_both = s0.grad.cpu().numpy().astype(np.float64)
+        gE_both = [float(e.grad) for e in E]
+    g_sep, gE_sep = [], []
+    for i in range(2):
+        with MPMSimulation(p, MaterialModel.kJelly, res=64, E=5e3) as sim:
+            s1 = to_state(p).requires_grad_(True)
+            e1 = torch.tensor([Es[i]], dtype=torch.float64, requires_grad=True)
+            (autodiff.rollout(sim, s1, e1, steps, every) * w[i]).sum().backward()
+            g_sep.append(s1.grad.cpu().numpy().astype(np.float64))
+            gE_sep.append(float(e1.grad))
+    assert gE_both == gE_sep, (gE_both, gE_sep)
+    np.testing.assert_allclose(g_both, g_sep[0] + g_sep[1], rtol=1e-5, atol=1e-6 * np.abs(g_both).max())
+
+
 def com_loss(out, d, target, n_per_scene):
     x = autodiff.unpack(out, d)[0]
     loss = 0.0

@@ -135,5 +135,69 @@ def test_batch_errors_and_scene_params():
         assert np.isclose(p["mu_0"], np.float32(2e3) / (np.float32(2) * np.float32(1.2)))
         assert p["dx"] == np.float32(1 / 32)
         b.advance(1)
-        with pytest.raises(IndexError):
+        with pytest.raises(IndexError, match="scene 1 "):
             b.particles()
+
+
+def reflection_scene():
+    """Two 2D snow particles, one with F a scaled reflection: the reference's closed-form polar
+    divides by zero there, so its first step leaves the particle at a NaN position and the second
+    step's grid index of that position throws (src/nclr.h:115, 163-164)."""
+    c, s = np.float32(0.6), np.float32(0.8)
+    p = Particles.from_positions(np.array([[0.5, 0.5], [0.52, 0.5]], np.float32))
+    p.F[0] = (np.float32(0.975) * np.array([[c, s], [s, -c]], np.float32)).astype(np.float32)
+    return p
+
+
+def assert_same_nan(a, b, label):
+    for k in FIELDS:
+        assert np.array_equal(getattr(a, k), getattr(b, k), equal_nan=True), f"{label}: field {k} differs"
+
+
+def test_failed_scenes_stop_alone_and_the_others_match_their_plain_runs():
+    """Each scene of a batch fails on its own, as each MPMSimulation of the reference throws on its
+    own: the step that finds a particle of scene s off its grid leaves scene s as it was before that
+    step, reports s (and the step) at the next synchronising call, and every later step reports it
+    again; the other scenes step on bit for bit like their plain runs.  Setting the state restarts
+    every scene."""
+    outside = Particles.from_positions(np.array([[0.5, 0.5], [1.2, 0.5], [0.3, 0.6]], np.float32))
+    specs = [SceneSpec(scenes.two_cubes_2d().particles, MaterialModel.kSnow, 64),
+             SceneSpec(reflection_scene(), MaterialModel.kSnow, 64),
+             SceneSpec(scenes.batched_2d(seed=5, scenes=3)[2].particles, MaterialModel.kJelly, 128),
+             SceneSpec(outside, MaterialModel.kJelly, 32)]
+    steps = 30
+    print(f"[batch] failure isolation: {len(specs)} scenes, {steps} steps", flush=True)
+    with MPMBatch(specs) as b:
+        b.advance(steps)
+        with pytest.raises(IndexError) as exc:
+            b.synchronize()
+        msg = str(exc.value)
+        print(f"[batch] reported: {msg}", flush=True)
+        assert "scene 1 (step 2), 3 (step 1)" in msg, msg
+        assert b.scene_status() == {1: {"flags": 1, "step": 2}, 3: {"flags": 1, "step": 1}}
+        b.synchronize()  # reported once per finding
+        outs = [b.particles(s).copy() for s in range(len(specs))]
+        grids = [b.grid(s) for s in range(len(specs))]
+        b.advance(5)
+        with pytest.raises(IndexError, match=r"scene 1 \(step 2\), 3 \(step 1\)"):
+            b.synchronize()
+        later = [b.particles(s).copy() for s in range(len(specs))]
+        b.set_particles(Particles.concatenate([sp.particles for sp in specs]))
+        assert b.scene_status() == {}
+        b.advance(1)
+        assert b.scene_status() == {3: {"flags": 1, "step": 36}}
+    for s, sp in enumerate(specs):
+        with MPMSimulation(sp.particles, sp.material, res=sp.res) as sim:
+            for total, got in ((steps, outs[s]), (steps + 5, later[s])):
+                sim.advance(total - sim.steps_taken)
+                try:
+                    sim.synchronize()
+                except IndexError:
+                    assert s in (1, 3)
+                assert_same_nan(got, sim.particles(), f"scene {s} after {total} steps")
+                if total == steps:
+                    gv, gm = sim.grid()
+                    assert np.array_equal(grids[s][0], gv, equal_nan=True) and np.array_equal(grids[s][1], gm), s
+    assert not np.isfinite(outs[1].x[0]).all()
+    assert_same_nan(outs[3], specs[3].particles, "scene 3 stays at its initial state")
+    assert not grids[3][1].any()  # stopped before completing a step: empty grid

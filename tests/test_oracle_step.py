@@ -263,10 +263,10 @@ def test_fp32_tracks_fp64_over_a_short_run():
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_parallel_step_matches_serial(dim):
-    """The OpenMP step of the CPU baseline differs from the serial one only in P2G's summation
-    order: after 3 steps every field agrees to FP32 rounding (the one-step parity metric, 1e-5)."""
-    from parity import assert_parity
-
+    """The OpenMP step (the multi-core CPU baseline and the FP64 finite-difference rollouts) is
+    bitwise the serial step: each thread adds the contributions that land in its band of grid rows
+    in particle order.  FP32 fast build and exact FP64 build, every material, with thread counts
+    that split the grid unevenly."""
     side = 20 if dim == 2 else 8
     g = np.linspace(0.3, 0.45, side, dtype=np.float32)
     a = np.stack(np.meshgrid(*([g] * dim), indexing="ij"), -1).reshape(-1, dim)
@@ -279,6 +279,20 @@ def test_parallel_step_matches_serial(dim):
     st = O.State(x=x, v=v, F=F, C=np.zeros_like(F), Jp=np.ones(n, np.float32), mass=np.ones(n, np.float32),
                  volume=np.ones(n, np.float32))
     for mat in (O.SNOW, O.JELLY, O.LIQUID):
-        ser = O.run_threads(st, mat, res=32, steps=3, identity_mode=1, threads=1)
-        par = O.run_threads(st, mat, res=32, steps=3, identity_mode=1, threads=4)
-        assert_parity(par, ser, inv_dx=32.0, label=f"OpenMP oracle, material {mat}")
+        for fp64 in (False, True):
+            ser = O.run_threads(st, mat, res=32, steps=3, identity_mode=1, threads=1, fp64=fp64)
+            for threads in (3, 7):
+                par = O.run_threads(st, mat, res=32, steps=3, identity_mode=1, threads=threads, fp64=fp64)
+                for k in ("x", "v", "F", "C", "Jp"):
+                    np.testing.assert_array_equal(getattr(par, k), getattr(ser, k), err_msg=f"{mat} {k} {fp64}")
+    # the exact FP64 build's threaded step is the exact serial step
+    ref = O.run(st, O.JELLY, res=32, steps=3, identity_mode=1, fp64=True)
+    par = O.run_threads(st, O.JELLY, res=32, steps=3, identity_mode=1, threads=5, fp64=True)
+    np.testing.assert_array_equal(par.x, ref.x)
+
+
+def test_parallel_step_raises_out_of_range():
+    x = np.array([[0.5, 0.5], [1.2, 0.5], [0.3, 0.3]], np.float32)
+    st = O.default_state(x)
+    with pytest.raises(O.OracleOutOfRange):
+        O.run_threads(st, O.SNOW, res=16, steps=1, threads=4, fp64=True)

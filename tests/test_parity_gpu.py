@@ -113,9 +113,24 @@ def test_full_size_four_cubes_3d():
     ref, floor = reorder_noise(to_oracle(s), lambda st: O.run(st, int(sc.material), res=sc.res, steps=1),
                                inv_dx=float(sc.res))
     errs = assert_parity_with_floor(g, ref, floor, inv_dx=float(sc.res), label="four_cubes_3d after 10")
-    print(f"[parity] four_cubes_3d after 10: errors {errs} reorder floor {floor}", flush=True)
-    for k in ("x", "v", "C", "Jp"):
+    above = {k: errs[k] for k in ("x", "v", "C", "F", "Jp") if errs[k] > TOL}
+    print(f"[parity] four_cubes_3d after 10: errors {errs} reorder floor {floor}; "
+          f"fields above {TOL} (within 2x the floor): {above}", flush=True)
+    # only F (the signed-SVD clamp) and C (its grid sums) may use the floor
+    for k in ("x", "v", "Jp"):
         assert errs[k] <= TOL, (k, errs[k])
+
+
+def test_full_size_four_cubes_3d_physical_identity():
+    """The same 1,048,576-particle 3D snow scene with the physical identity F0 = I: the cubes stay
+    compact and elastic-plastic for hundreds of steps, so a mid-trajectory state is not chaotic and
+    every field is held to the strict 1e-5 after one step (from the state after 30 engine steps)."""
+    sc = scenes.four_cubes_3d()
+    p = Particles.from_positions(sc.particles.x, v=sc.particles.v, identity_mode=1)
+    s = gpu_state(p, sc.material, sc.res, 30, identity_mode=1)
+    print(f"[parity] four_cubes_3d physical identity after 30: n={s.n} Jp range "
+          f"[{s.Jp.min():.4f}, {s.Jp.max():.4f}]", flush=True)
+    one_step_case(s, sc.material, sc.res, identity_mode=1, label="four_cubes_3d physical after 30")
 
 
 def test_empty_and_single_particle():
@@ -156,9 +171,16 @@ def test_out_of_range_raises():
         O.run(to_oracle(p), 0, res=16, steps=1)
     with MPMSimulation(p, MaterialModel.kSnow, res=16) as sim:
         sim.advance(1)
-        with pytest.raises(IndexError):
+        with pytest.raises(IndexError, match="in step 1"):
             sim.particles()
-        # the error is reported once; a fresh simulation of in-range particles works
+        # reported once per finding; the simulation stays at the state before the failing step
+        np.testing.assert_array_equal(sim.particles().x, p.x)
+        sim.advance(3)
+        with pytest.raises(IndexError, match="in step 1"):
+            sim.synchronize()
+        np.testing.assert_array_equal(sim.particles().x, p.x)
+        assert sim.steps_taken == 4
+    # a fresh simulation of in-range particles works
     ok = Particles.from_positions(np.array([[0.5, 0.5]], np.float32))
     with MPMSimulation(ok, MaterialModel.kSnow, res=16) as sim:
         sim.advance(2)
