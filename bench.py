@@ -327,6 +327,10 @@ def main() -> int:
                     help="replay each step as a captured hipGraph (measured slower than direct launches on this "
                          "ROCm stack for every workload here, so off by default)")
     ap.add_argument("--fast-svd", action="store_true", help="engine option fast_svd (not bitwise the reference)")
+    ap.add_argument("--svd", choices=["exact", "fast", "accurate"], default=None,
+                    help="SVD mode of the step: exact (nclr_svd's FP32 operation order, bitwise the reference's "
+                         "arithmetic), fast (FP32 Jacobi of A^T A) or accurate (FP64 Newton polar factor and FP64 "
+                         "Jacobi SVD)")
     ap.add_argument("--tuning", default="auto",
                     help="P2G particles per work item and G2P workgroup size as 'CHUNKxTHREADS' (e.g. 512x256), "
                          "or 'auto' to let the engine's autotuner time every shape on the warmed-up state")
@@ -361,8 +365,9 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
+    svd = args.svd or ("fast" if args.fast_svd else "exact")
     sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity,
-                        graph=args.graph, fast_svd=args.fast_svd)
+                        graph=args.graph, svd=svd)
     sim.advance(args.warmup)
     sim.synchronize()
     if args.tuning == "auto":
@@ -438,7 +443,10 @@ def main() -> int:
             "hipgraph": args.graph,
             "tuning": {"p2g_chunk": int(tuning["p2g_chunk"]), "g2p_threads": int(tuning["g2p_threads"]),
                        "chosen_by": "autotuner" if args.tuning == "auto" else "flag"},
-            "svd": "fast (not bitwise the reference)" if args.fast_svd else "nclr_svd order (bitwise the reference's FP32)",
+            "svd": {"exact": "nclr_svd order (bitwise the reference's FP32)",
+                    "fast": "fast FP32 Jacobi of A^T A (not bitwise the reference)",
+                    "accurate": "FP64 Newton polar factor and FP64 Jacobi SVD (closer to the FP64 step than the "
+                                "reference's FP32 arithmetic)"}[svd],
         },
         "roofline": {
             "kernel": dom,

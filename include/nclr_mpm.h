@@ -61,11 +61,15 @@ typedef struct mpm_options {
     int32_t autotune;
     /* 3D singular value decompositions (the polar factor in P2G, the snow / liquid SVD in G2P):
      * 0 = nclr_svd's two-sided Jacobi in Eigen's operation order (src/nclr_math.h:50-74), bitwise
-     * the CPU restatement's FP32 arithmetic; 1 = a fixed-sweep symmetric Jacobi SVD of A^T A with
-     * the same convention (U, V proper rotations, descending order, the sign of det on the last
-     * singular value): cheaper, but its rotation factor carries ~1e-7 absolute error where the
-     * two-sided method is exact to rounding of the (small) strain, which the elastic stress
-     * (F - R) amplifies at small strains. */
+     * the CPU restatement's FP32 arithmetic (the default; the drop-in's mode);
+     * 1 = a fixed-sweep FP32 symmetric Jacobi SVD of A^T A with the same convention (U, V proper
+     * rotations, descending order, the sign of det on the last singular value): cheaper, but its
+     * rotation factor carries ~1e-7 absolute error, which the elastic stress (F - R) amplifies at
+     * small strains;
+     * 2 = accurate: F - R by Newton's polar iteration in FP64 (two-sided Jacobi where det F <= 0 or
+     * F is ill-conditioned) and the snow / liquid SVD by an FP64 Jacobi eigen-decomposition of
+     * A^T A; closer to the FP64 step than the reference's FP32 arithmetic is, and faster than 0.
+     * The adjoint kernels use mode 0's factors for modes 0 and 2. */
     int32_t fast_svd;
     int32_t reserved[8];
 } mpm_options;
@@ -172,10 +176,12 @@ int mpm_active_nodes_scene(mpm_handle h, int scene, int64_t* count);
  * counterpart; the reference steps particles in index order, src/nclr.h:112,169). */
 int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int device);
 
+
 /* The step's device SVD on n host matrices (dim x dim, row-major): A = U diag(sigma) V^T with
  * nclr_svd's convention (src/nclr_math.h:50-74: descending order; in 3D U, V proper rotations and
- * the sign of det A on sigma[2]).  exact = 1: nclr_svd's operation order, 0: the fast path of mpm_options.fast_svd;
- * 2D always uses it. */
+ * the sign of det A on sigma[2]).  exact = 1: nclr_svd's operation order, 0: the fast FP32 path
+ * (mpm_options.fast_svd = 1), 2: the accurate FP64 path (mpm_options.fast_svd = 2); 2D always uses
+ * nclr_svd's order. */
 int mpm_svd(int dim, int64_t n, const float* A, float* U, float* sigma, float* V, int exact, int device);
 
 /* ---- Differentiation (no reference counterpart: the reference has no adjoint) ----------------

@@ -205,7 +205,7 @@ NCLR_DI void material_backward(int material, const M<D>& Fh, float Jp, const M<D
     }
     M<D> U, V;
     float sig[D];
-    step_svd<EXACT, D>(Fh, U, sig, V);
+    step_svd<EXACT ? kSvdExact : kSvdFast, D>(Fh, U, sig, V);
     if (material == MPM_SNOW) {
         const float lo = float(1.0 - 2.5e-2), hi = float(1.0 + 4.5e-3);
         M<D> US;
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     float dw[D][3];
 #pragma unroll
     for (int a = 0; a < D; ++a) bspline_dweights(fx[a], dw[a]);
-    const M<D> A = fused_affine<D, EXACT>(sp.material, F, C, Jp, mass, vol, sp);
+    const M<D> A = fused_affine<D, EXACT ? kSvdExact : kSvdFast>(sp.material, F, C, Jp, mass, vol, sp);
     M<D> Abar;
     float fxbar[D], vin[D];
 #pragma unroll
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             sG += G.a[a][b];
         }
     // PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry
-    const M<D> R = polar_rotation_mode<EXACT>(F);
+    const M<D> R = polar_rotation_mode<EXACT ? kSvdExact : kSvdFast>(F);
     const float J = det(F);
     M<D> Fbar, Rbar;
     float mubar = 0.0f;
@@ -912,7 +912,7 @@ template <int D, int PPT>
 static hipError_t g2p_adjoint_material(const LaunchArgs& a, const AdjointArgs& b) {
     if (a.material == MPM_JELLY) return g2p_adjoint_one<D, PPT, false, true>(a, b);
     // 2D SVDs have one method; in 3D the forward's choice
-    if (D == 2 || a.exact_svd) return g2p_adjoint_one<D, PPT, true, true>(a, b);
+    if (D == 2 || (a.svd_mode != 1)) return g2p_adjoint_one<D, PPT, true, true>(a, b);
     return g2p_adjoint_one<D, PPT, true, false>(a, b);
 }
 
@@ -945,7 +945,7 @@ hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
         if (a.dim == 2)
             k_p2g_adjoint<2, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
                 a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
-        else if (a.exact_svd)
+        else if ((a.svd_mode != 1))
             k_p2g_adjoint<3, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
                 a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
         else

@@ -74,6 +74,10 @@ bool valid_threads(int t) { return t == 64 || t == 128 || t == 256; }
 
 }  // namespace
 
+namespace nclr_host {
+int set_last_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace nclr_host
+
 struct nclr_mpm_sim {
     int dim = 0;
     int material = 0;  // common material of all scenes or nclr_dev::kMixed
@@ -133,7 +137,7 @@ struct nclr_mpm_sim {
         a.dim = dim;
         a.material = material;
         a.g2p_threads = g2p_threads;
-        a.exact_svd = opts.fast_svd ? 0 : 1;
+        a.svd_mode = opts.fast_svd;
         a.p = p;
         a.stamp = stamp;
         a.state_in = state[parity];
@@ -749,6 +753,8 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
     if (n > (int64_t(1) << 31) - 64) return fail(MPM_ERR_INVALID, "particle count out of range");
     if (n > 0 && !x) return fail(MPM_ERR_INVALID, "x must not be NULL");
 
+    if (opts && (opts->fast_svd < 0 || opts->fast_svd > 2))
+        return fail(MPM_ERR_INVALID, "fast_svd must be 0 (exact), 1 (fast) or 2 (accurate)");
     auto* h = new nclr_mpm_sim();
     if (opts)
         h->opts = *opts;
@@ -1133,6 +1139,7 @@ int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int
 int mpm_svd(int dim, int64_t n, const float* A, float* U, float* sigma, float* V, int exact, int device) {
     if (dim != 2 && dim != 3) return fail(MPM_ERR_INVALID, "dim must be 2 or 3");
     if (n < 0 || (n > 0 && (!A || !U || !sigma || !V))) return fail(MPM_ERR_INVALID, "bad arguments");
+    if (exact < 0 || exact > 2) return fail(MPM_ERR_INVALID, "exact must be 1 (exact), 0 (fast) or 2 (accurate)");
     if (n == 0) return MPM_OK;
     HIP_TRY(hipSetDevice(device));
     hipStream_t s = nullptr;
@@ -1144,7 +1151,7 @@ int mpm_svd(int dim, int64_t n, const float* A, float* U, float* sigma, float* V
     if (e == hipSuccess) e = dmalloc(&dS, vecs);
     if (e == hipSuccess) e = dmalloc(&dV, mats);
     if (e == hipSuccess) e = hipMemcpyAsync(dA, A, mats * sizeof(float), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = nclr_host::launch_svd(dim, exact != 0, dA, dU, dS, dV, n, s);
+    if (e == hipSuccess) e = nclr_host::launch_svd(dim, exact == 1 ? 0 : exact == 0 ? 1 : 2, dA, dU, dS, dV, n, s);
     if (e == hipSuccess) e = hipMemcpyAsync(U, dU, mats * sizeof(float), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(sigma, dS, vecs * sizeof(float), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(V, dV, mats * sizeof(float), hipMemcpyDeviceToHost, s);

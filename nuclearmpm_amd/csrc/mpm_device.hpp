@@ -84,22 +84,23 @@ NCLR_DI float hardening(int material, float Jp) {
 }
 
 // first_piola_kirchoff_stress (src/nclr.h:313-337): -(dt V0)(4 inv_dx^2 PF) + m C with
-// PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry.
-template <int D, bool EXACT>
+// PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry.  MODE: the SVD mode of the polar factor
+// (kSvdExact: the reference's float arithmetic bit for bit).
+template <int D, int MODE>
 NCLR_DI M<D> fused_affine(int material, const M<D>& F, const M<D>& C, float Jp, float mass, float vol,
                           const SceneParams& sp) {
     const float e = hardening(material, Jp);
     const float mu = sp.mu0 * e;
     const float lambda = sp.lambda0 * e;
     const float J = det(F);
-    const M<D> R = polar_rotation_mode<EXACT>(F);
+    const M<D> diff = f_minus_rotation<MODE, D>(F);
     const float Dinv = 4.0f * sp.inv_dx * sp.inv_dx;
     const float two_mu = 2.0f * mu;
     M<D> FmR;
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-        for (int b = 0; b < D; ++b) FmR.a[a][b] = two_mu * (F.a[a][b] - R.a[a][b]);
+        for (int b = 0; b < D; ++b) FmR.a[a][b] = two_mu * diff.a[a][b];
     const M<D> PF = matmul_bt<D>(FmR, F);
     const float lam_term = lambda * (J - 1.0f) * J;
     const float neg_dtv = -(sp.dt * vol);

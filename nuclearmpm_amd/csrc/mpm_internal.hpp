@@ -87,7 +87,8 @@ struct LaunchArgs {
     int dim;
     int material;  // common material of all scenes, or nclr_dev::kMixed
     int g2p_threads;
-    int exact_svd;  // 1: 3D SVDs in nclr_svd's operation order (bitwise the oracle), 0: fast path
+    int svd_mode;   // 3D SVDs and polar factors: 0 nclr_svd's operation order (bitwise the oracle), 1 fast
+                    // FP32 path, 2 accurate FP64 path (the adjoint kernels take nclr_svd's order for 0 and 2)
     nclr_dev::StepParams p;
     uint32_t* stamp;           // device stamp word (same as p.stamp)
     const float* state_in;     // SoA state, current physical order
@@ -151,7 +152,7 @@ hipError_t launch_grid(const LaunchArgs& a);
 hipError_t launch_g2p(const LaunchArgs& a);
 hipError_t launch_export_grid(const LaunchArgs& a, int scene, int res, uint32_t node_tile_base, int ntn,
                               float* vel, float* mass, int64_t total);
-hipError_t launch_svd(int dim, bool exact, const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s);
+hipError_t launch_svd(int dim, int mode, const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t s);
 hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
 hipError_t launch_bump_u32(uint32_t* p, hipStream_t s);  // *p += 1 on the device
@@ -176,6 +177,8 @@ hipError_t launch_gyroid_scatter(const mpm_cube& box, float k, float t, float th
                                  int identity_mode, int64_t n, float* img, hipStream_t s);
 hipError_t launch_project(int64_t n, const float* pos, int centered, float* out, hipStream_t s);
 int64_t max_items(int64_t n, uint32_t ntiles);
+// Records msg as the calling thread's mpm_last_error() and returns code.
+int set_last_error(int code, const char* msg);
 int num_fields(int dim);
 int region_nodes(int dim);
 int tile_nodes(int dim);

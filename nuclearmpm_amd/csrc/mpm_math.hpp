@@ -396,11 +396,235 @@ NCLR_DI void fast_svd3(const M<3>& A, M<3>& U, float (&sig)[3], M<3>& V) {
     sig[2] = fmaf(u3[0], av[2][0], fmaf(u3[1], av[2][1], u3[2] * av[2][2]));
 }
 
-// The SVD of the step: nclr_svd's operation order when EXACT (bitwise the oracle's FP32
-// arithmetic), else the fast 3x3 path (2x2 always takes nclr_svd, a single Jacobi pair).
-template <bool EXACT, int D>
+// ---------------------------------------------------------------------------------------------
+// Accurate mode: the 3x3 polar factor and SVD in double precision, more accurate than the FP32
+// two-sided Jacobi (whose R carries the FP32 rounding of its rotations, amplified by the stress's
+// F - R at small strain).  FP64 FMA, add and multiply issue at the FP32 rate on gfx950; the
+// reciprocal, square root and reciprocal square root estimates take one Newton step each.
+// ---------------------------------------------------------------------------------------------
+NCLR_DI double rcp_f64(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, r, 1.0);
+    const double r1 = fma(r, e, r);
+    return fma(r1, fma(-x, r1, 1.0), r1);
+}
+NCLR_DI double rsq_f64(double x) {  // x > 0
+    const double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x * y;
+    const double y1 = fma(y, fma(-h, y, 0.5), y);  // y (1.5 - x y^2 / 2)
+    return fma(y1, fma(-0.5 * x * y1, y1, 0.5), y1);
+}
+
+NCLR_DI void cofactors(const double (&X)[3][3], double (&c)[3][3]) {
+    c[0][0] = fma(X[1][1], X[2][2], -X[1][2] * X[2][1]);
+    c[0][1] = fma(X[1][2], X[2][0], -X[1][0] * X[2][2]);
+    c[0][2] = fma(X[1][0], X[2][1], -X[1][1] * X[2][0]);
+    c[1][0] = fma(X[0][2], X[2][1], -X[0][1] * X[2][2]);
+    c[1][1] = fma(X[0][0], X[2][2], -X[0][2] * X[2][0]);
+    c[1][2] = fma(X[0][1], X[2][0], -X[0][0] * X[2][1]);
+    c[2][0] = fma(X[0][1], X[1][2], -X[0][2] * X[1][1]);
+    c[2][1] = fma(X[0][2], X[1][0], -X[0][0] * X[1][2]);
+    c[2][2] = fma(X[0][0], X[1][1], -X[0][1] * X[1][0]);
+}
+
+// F - R, R the polar rotation of F (the reference's nclr_polar, src/nclr_math.h:76-98, for det F
+// > 0), by Newton's iteration X <- (X + X^-T) / 2 = (X + cof(X) / det X) / 2 from X = F in double,
+// for F with ||F||_F^3 <= 40 det F (condition number below ~8; false otherwise, and the caller
+// takes the Jacobi SVD).  The iteration converges quadratically to R; once a step changes X by at
+// most 1e-6 one more step leaves an error below 1e-12, and F - R is rounded once to float, so it
+// keeps its relative precision at small strain.
+NCLR_DI bool f_minus_polar_newton(const M<3>& F, M<3>& out) {
+    double X[3][3], c[3][3];
+    double nf2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            X[i][j] = double(F.a[i][j]);
+            nf2 = fma(X[i][j], X[i][j], nf2);
+        }
+    cofactors(X, c);
+    double det = fma(X[0][0], c[0][0], fma(X[0][1], c[0][1], X[0][2] * c[0][2]));
+    if (!(det > 0.0) || !(nf2 * nf2 * nf2 <= 1600.0 * det * det)) return false;
+    bool conv = false;
+    for (int it = 0; it < 10; ++it) {
+        const double half_inv = 0.5 * rcp_f64(det);
+        double mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double xn = fma(c[i][j], half_inv, 0.5 * X[i][j]);
+                mx = fmax(mx, fabs(xn - X[i][j]));
+                X[i][j] = xn;
+            }
+        cofactors(X, c);
+        det = fma(X[0][0], c[0][0], fma(X[0][1], c[0][1], X[0][2] * c[0][2]));
+        if (mx <= 1e-6) {
+            conv = true;
+            break;
+        }
+    }
+    if (!conv) return false;
+    const double half_inv = 0.5 * rcp_f64(det);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out.a[i][j] = float(double(F.a[i][j]) - fma(c[i][j], half_inv, 0.5 * X[i][j]));
+    return true;
+}
+
+// One rotation of the cyclic Jacobi eigen-decomposition of the symmetric B (Golub & Van Loan,
+// Matrix Computations, Algorithm 8.5.2 with the symmetric Schur step 8.5.1) in double,
+// accumulated into the columns P, Q of W.
+template <int P, int Q>
+NCLR_DI void sym_jacobi_f64(double (&B)[3][3], double (&W)[3][3]) {
+    constexpr int R = 3 - P - Q;
+    const double bpq = B[P][Q];
+    if (bpq == 0.0) return;
+    const double bpp = B[P][P], bqq = B[Q][Q];
+    const double tau = (bqq - bpp) * rcp_f64(2.0 * bpq);
+    const double root = fma(tau, tau, 1.0);
+    // |tau| above 1e150: t = 1 / (2 tau) to double precision, and tau^2 would overflow
+    const double t = fabs(tau) > 1e150 ? 0.5 * rcp_f64(tau) : copysign(1.0, tau) * rcp_f64(fabs(tau) + root * rsq_f64(root));
+    const double c = rsq_f64(fma(t, t, 1.0));
+    const double s = t * c;
+    B[P][P] = fma(-t, bpq, bpp);
+    B[Q][Q] = fma(t, bpq, bqq);
+    B[P][Q] = B[Q][P] = 0.0;
+    const double brp = B[R][P], brq = B[R][Q];
+    B[R][P] = B[P][R] = fma(c, brp, -s * brq);
+    B[R][Q] = B[Q][R] = fma(s, brp, c * brq);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double wp = W[k][P], wq = W[k][Q];
+        W[k][P] = fma(c, wp, -s * wq);
+        W[k][Q] = fma(s, wp, c * wq);
+    }
+}
+
+// Orders eigenpairs I < J by decreasing eigenvalue; a swap negates one column (det W stays +1).
+template <int I, int J>
+NCLR_DI void sort_pair_f64(double (&l)[3], double (&W)[3][3]) {
+    const bool sw = l[I] < l[J];
+    const double li = l[I], lj = l[J];
+    l[I] = sw ? lj : li;
+    l[J] = sw ? li : lj;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double wi = W[k][I], wj = W[k][J];
+        W[k][I] = sw ? wj : wi;
+        W[k][J] = sw ? -wi : wj;
+    }
+}
+
+// 3x3 SVD with nclr_svd's convention (U, V proper rotations, singular values in descending order
+// of magnitude, the sign of det A on the last one), in double: Jacobi eigen-decomposition of
+// A^T A until the off-diagonal part is below 1e-15 of the diagonal (at most 8 sweeps), then
+// u_1 = A v_1 / |A v_1|, u_2 from A v_2 by Gram-Schmidt, u_3 = u_1 x u_2 and sigma_i = u_i^T A v_i,
+// rounded to float.  The singular vectors of a small singular value come from the eigenvectors of
+// A^T A and the cross product, so rank-deficient inputs (the reference's F = diag(1, 1, 0) in 3D)
+// keep orthonormal factors.  Non-finite input gives NaN factors, like nclr_svd.
+NCLR_DI void accurate_svd3(const M<3>& A, M<3>& U, float (&sig)[3], M<3>& V) {
+    double a[3][3], B[3][3], W[3][3];
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            a[i][j] = double(A.a[i][j]);
+            finite = finite && isfinite(A.a[i][j]);
+            W[i][j] = i == j ? 1.0 : 0.0;
+        }
+    if (!finite) {
+        const float nan = __builtin_nanf("");
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            sig[i] = nan;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) U.a[i][j] = V.a[i][j] = nan;
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            const double v = fma(a[0][i], a[0][j], fma(a[1][i], a[1][j], a[2][i] * a[2][j]));
+            B[i][j] = v;
+            B[j][i] = v;
+        }
+    for (int sweep = 0; sweep < 8; ++sweep) {
+        const double off = fma(B[0][1], B[0][1], fma(B[0][2], B[0][2], B[1][2] * B[1][2]));
+        const double diag = fma(B[0][0], B[0][0], fma(B[1][1], B[1][1], B[2][2] * B[2][2]));
+        if (off <= 1e-30 * diag) break;
+        sym_jacobi_f64<0, 1>(B, W);
+        sym_jacobi_f64<0, 2>(B, W);
+        sym_jacobi_f64<1, 2>(B, W);
+    }
+    double l[3] = {B[0][0], B[1][1], B[2][2]};
+    sort_pair_f64<0, 1>(l, W);
+    sort_pair_f64<0, 2>(l, W);
+    sort_pair_f64<1, 2>(l, W);
+    double av[3][3];  // av[i] = A v_i
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) av[i][r] = fma(a[r][0], W[0][i], fma(a[r][1], W[1][i], a[r][2] * W[2][i]));
+    double u1[3], u2[3], u3[3];
+    {
+        const double n2 = fma(av[0][0], av[0][0], fma(av[0][1], av[0][1], av[0][2] * av[0][2]));
+        const bool ok = n2 > 0.0;
+        const double inv = ok ? rsq_f64(n2) : 0.0;
+        u1[0] = ok ? av[0][0] * inv : 1.0;
+        u1[1] = ok ? av[0][1] * inv : 0.0;
+        u1[2] = ok ? av[0][2] * inv : 0.0;
+    }
+    {
+        const double d = fma(u1[0], av[1][0], fma(u1[1], av[1][1], u1[2] * av[1][2]));
+        double w[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) w[r] = fma(-d, u1[r], av[1][r]);
+        double n2 = fma(w[0], w[0], fma(w[1], w[1], w[2] * w[2]));
+        if (!(n2 > 1e-60 * fma(av[1][0], av[1][0], fma(av[1][1], av[1][1], fma(av[1][2], av[1][2], 1e-300))))) {
+            // rank <= 1: any unit vector orthogonal to u1
+            const bool ex = fabs(u1[0]) < 0.9;
+            const double e[3] = {ex ? 1.0 : 0.0, ex ? 0.0 : 1.0, 0.0};
+            const double de = fma(u1[0], e[0], u1[1] * e[1]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) w[r] = fma(-de, u1[r], e[r]);
+            n2 = fma(w[0], w[0], fma(w[1], w[1], w[2] * w[2]));
+        }
+        const double inv = rsq_f64(n2);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) u2[r] = w[r] * inv;
+    }
+    u3[0] = fma(u1[1], u2[2], -u1[2] * u2[1]);
+    u3[1] = fma(u1[2], u2[0], -u1[0] * u2[2]);
+    u3[2] = fma(u1[0], u2[1], -u1[1] * u2[0]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        U.a[r][0] = float(u1[r]);
+        U.a[r][1] = float(u2[r]);
+        U.a[r][2] = float(u3[r]);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) V.a[r][cc] = float(W[r][cc]);
+    }
+    sig[0] = float(fma(u1[0], av[0][0], fma(u1[1], av[0][1], u1[2] * av[0][2])));
+    sig[1] = float(fma(u2[0], av[1][0], fma(u2[1], av[1][1], u2[2] * av[1][2])));
+    sig[2] = float(fma(u3[0], av[2][0], fma(u3[1], av[2][1], u3[2] * av[2][2])));
+}
+
+// SVD modes of the step: 0 nclr_svd's operation order (bitwise the oracle's FP32 arithmetic),
+// 1 the fast FP32 path, 2 the accurate FP64 path (2x2 always takes nclr_svd, one Jacobi pair).
+constexpr int kSvdExact = 0;
+constexpr int kSvdFast = 1;
+constexpr int kSvdAccurate = 2;
+
+template <int MODE, int D>
 NCLR_DI void step_svd(const M<D>& A, M<D>& U, float (&sig)[D], M<D>& V) {
-    if constexpr (D == 3 && !EXACT) fast_svd3(A, U, sig, V);
+    if constexpr (D == 3 && MODE == kSvdFast) fast_svd3(A, U, sig, V);
+    else if constexpr (D == 3 && MODE == kSvdAccurate) accurate_svd3(A, U, sig, V);
     else nclr_svd<D>(A, U, sig, V);
 }
 
@@ -418,21 +642,39 @@ NCLR_DI M<2> polar_rotation(const M<2>& A) {
     return R;
 }
 
-template <bool EXACT = true>
+template <int MODE = kSvdExact>
 NCLR_DI M<3> polar_rotation(const M<3>& A) {
     M<3> U, V;
     float sig[3];
-    step_svd<EXACT, 3>(A, U, sig, V);
+    step_svd<MODE, 3>(A, U, sig, V);
     return matmul_bt<3>(U, V);
 }
 
-template <bool EXACT>
+template <int MODE>
 NCLR_DI M<2> polar_rotation_mode(const M<2>& A) {
     return polar_rotation(A);
 }
-template <bool EXACT>
+template <int MODE>
 NCLR_DI M<3> polar_rotation_mode(const M<3>& A) {
-    return polar_rotation<EXACT>(A);
+    return polar_rotation<MODE>(A);
+}
+
+// F - R with R = nclr_polar's rotation factor.  The accurate mode takes Newton's iteration where F
+// is well-conditioned with det F > 0, and the two-sided Jacobi SVD elsewhere (in the accurate
+// mode the Jacobi result's F - R is that of nclr_svd, so the fallback keeps the reference's
+// rotation convention for det F <= 0).
+template <int MODE, int D>
+NCLR_DI M<D> f_minus_rotation(const M<D>& F) {
+    M<D> out;
+    if constexpr (D == 3 && MODE == kSvdAccurate) {
+        if (f_minus_polar_newton(F, out)) return out;
+    }
+    const M<D> R = polar_rotation_mode<MODE == kSvdAccurate ? kSvdExact : MODE>(F);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) out.a[a][b] = F.a[a][b] - R.a[a][b];
+    return out;
 }
 
 // Quadratic B-spline weights (src/nclr.h:124-127).

@@ -208,7 +208,7 @@ NCLR_DI void p2g_deposit(unsigned long long* __restrict__ acc, uint32_t* __restr
         }
     }
 }
-template <int D, int MAT, int PPT, bool EXACT>
+template <int D, int MAT, int PPT, int MODE>
 __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, const uint32_t* __restrict__ perm,
                                                   const uint32_t* __restrict__ item_offset, StepParams p,
                                                   float4* __restrict__ slots) {
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
         float fx[D], w[D][3];
         if (!stencil<D>(x, sp, base, fx, w)) continue;  // keyed into this tile, so never taken
         valid[k] = true;
-        As[k] = fused_affine<D, EXACT>(material, F, C, Jp, mass, vol, sp);
+        As[k] = fused_affine<D, MODE>(material, F, C, Jp, mass, vol, sp);
         ms[k] = mass;
         float wsum = 1.0f;
 #pragma unroll
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t* __restrict__ v
 // ---------------------------------------------------------------------------------------------
 // G2P (src/nclr.h:167-261), reading slot perm[j] and writing slot j of the other buffer
 // ---------------------------------------------------------------------------------------------
-template <int D, int MAT, bool EXACT>
+template <int D, int MAT, int MODE>
 __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st, float* __restrict__ out,
                                                     const uint32_t* __restrict__ pid_in,
                                                     uint32_t* __restrict__ pid_out, const uint32_t* __restrict__ perm,
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(256) void k_g2p_sorted(const float* __restrict__ st
     } else {
         M<D> U, V;
         float sig[D];
-        step_svd<EXACT, D>(Fh, U, sig, V);
+        step_svd<MODE, D>(Fh, U, sig, V);
         if (material == MPM_SNOW) {
             const float lo = float(1.0 - 2.5e-2), hi = float(1.0 + 4.5e-3);
 #pragma unroll
@@ -655,7 +655,7 @@ __global__ __launch_bounds__(256) void k_export_grid(const float4* __restrict__ 
 }
 
 // The step's SVD on a batch of row-major matrices.
-template <int D, bool EXACT>
+template <int D, int MODE>
 __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ A, float* __restrict__ U,
                                              float* __restrict__ S, float* __restrict__ V, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -666,7 +666,7 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ A, float*
     for (int r = 0; r < D; ++r)
 #pragma unroll
         for (int c = 0; c < D; ++c) a.a[r][c] = A[(i * D + r) * D + c];
-    step_svd<EXACT, D>(a, u, s, v);
+    step_svd<MODE, D>(a, u, s, v);
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         S[i * D + r] = s[r];
@@ -744,13 +744,14 @@ static hipError_t sort_dispatch(const LaunchArgs& a) {
 template <int D, int MAT, int PPT>
 static hipError_t p2g_one(const LaunchArgs& a) {
     const int64_t items = (a.p.n + 256 * PPT - 1) / (256 * PPT) + a.p.ntiles;
-    if (D == 3 && !a.exact_svd) {
-        k_p2g_tile<D, MAT, PPT, false><<<unsigned(items), 256, 0, a.stream>>>(
-            a.state_in, a.perm, a.item_offset, a.p, a.slots);
-        return hipGetLastError();
-    }
-    k_p2g_tile<D, MAT, PPT, true><<<unsigned(items), 256, 0, a.stream>>>(a.state_in, a.perm, a.item_offset, a.p,
-                                                                   a.slots);
+    const unsigned g = unsigned(items);
+    if (D == 3 && a.svd_mode == kSvdFast)
+        k_p2g_tile<D, MAT, PPT, kSvdFast><<<g, 256, 0, a.stream>>>(a.state_in, a.perm, a.item_offset, a.p, a.slots);
+    else if (D == 3 && a.svd_mode == kSvdAccurate)
+        k_p2g_tile<D, MAT, PPT, kSvdAccurate><<<g, 256, 0, a.stream>>>(a.state_in, a.perm, a.item_offset, a.p,
+                                                                      a.slots);
+    else
+        k_p2g_tile<D, MAT, PPT, kSvdExact><<<g, 256, 0, a.stream>>>(a.state_in, a.perm, a.item_offset, a.p, a.slots);
     return hipGetLastError();
 }
 
@@ -777,13 +778,16 @@ static hipError_t p2g_dispatch(const LaunchArgs& a) {
 template <int D, int MAT>
 static hipError_t g2p_one(const LaunchArgs& a) {
     const int threads = a.g2p_threads > 0 ? a.g2p_threads : 256;
-    if (D == 3 && !a.exact_svd) {
-        k_g2p_sorted<D, MAT, false><<<xcd_grid(blocks_for(a.p.n, threads)), threads, 0, a.stream>>>(
-            a.state_in, a.state_out, a.pid_in, a.pid_out, a.perm, a.keys, a.p, a.gv);
-        return hipGetLastError();
-    }
-    k_g2p_sorted<D, MAT, true><<<xcd_grid(blocks_for(a.p.n, threads)), threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in,
-                                                                              a.pid_out, a.perm, a.keys, a.p, a.gv);
+    const unsigned g = xcd_grid(blocks_for(a.p.n, threads));
+    if (D == 3 && a.svd_mode == kSvdFast)
+        k_g2p_sorted<D, MAT, kSvdFast><<<g, threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in, a.pid_out,
+                                                                   a.perm, a.keys, a.p, a.gv);
+    else if (D == 3 && a.svd_mode == kSvdAccurate)
+        k_g2p_sorted<D, MAT, kSvdAccurate><<<g, threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in,
+                                                                       a.pid_out, a.perm, a.keys, a.p, a.gv);
+    else
+        k_g2p_sorted<D, MAT, kSvdExact><<<g, threads, 0, a.stream>>>(a.state_in, a.state_out, a.pid_in, a.pid_out,
+                                                                    a.perm, a.keys, a.p, a.gv);
     return hipGetLastError();
 }
 
@@ -829,12 +833,13 @@ hipError_t launch_export_grid(const LaunchArgs& a, int scene, int res, uint32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_svd(int dim, bool exact, const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s) {
+hipError_t launch_svd(int dim, int mode, const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const unsigned b = blocks_for(n, 256);
-    if (dim == 2) k_svd<2, true><<<b, 256, 0, s>>>(A, U, S, V, n);
-    else if (exact) k_svd<3, true><<<b, 256, 0, s>>>(A, U, S, V, n);
-    else k_svd<3, false><<<b, 256, 0, s>>>(A, U, S, V, n);
+    if (dim == 2) k_svd<2, kSvdExact><<<b, 256, 0, s>>>(A, U, S, V, n);
+    else if (mode == kSvdFast) k_svd<3, kSvdFast><<<b, 256, 0, s>>>(A, U, S, V, n);
+    else if (mode == kSvdAccurate) k_svd<3, kSvdAccurate><<<b, 256, 0, s>>>(A, U, S, V, n);
+    else k_svd<3, kSvdExact><<<b, 256, 0, s>>>(A, U, S, V, n);
     return hipGetLastError();
 }
 

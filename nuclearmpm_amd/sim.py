@@ -141,6 +141,20 @@ def _placeholder(n: int, dim: int) -> Particles:
     return Particles.from_positions(np.zeros((n, dim), np.float32))
 
 
+SVD_MODES = {"exact": 0, "fast": 1, "accurate": 2}
+
+
+def svd_mode(svd: Optional[str], fast_svd: bool = False) -> int:
+    """Engine option fast_svd from svd = "exact" (nclr_svd's FP32 operation order, bitwise the
+    reference's arithmetic; the default), "fast" (FP32 Jacobi of A^T A) or "accurate" (FP64 Newton
+    polar factor and FP64 Jacobi SVD); fast_svd=True is the older spelling of "fast"."""
+    if svd is None:
+        return 1 if fast_svd else 0
+    if svd not in SVD_MODES:
+        raise ValueError(f"svd must be one of {sorted(SVD_MODES)}")
+    return SVD_MODES[svd]
+
+
 def default_options(**overrides) -> mpm_options:
     o = mpm_options()
     lib().mpm_default_options(ctypes.byref(o))
@@ -157,13 +171,13 @@ class _Engine:
     _h = None
 
     def _init_common(self, dim: int, n: int, colors, identity_mode, p2g_chunk, g2p_threads, graph, autotune, timing,
-                     device, fast_svd=False):
+                     device, fast_svd=False, svd=None):
         self._dim = dim
         self._n = n
         self._colors = colors
         self.options = default_options(identity_mode=identity_mode, p2g_chunk=p2g_chunk, g2p_threads=g2p_threads,
                                        graph=int(graph), autotune=int(autotune), timing=int(timing), device=device,
-                                       fast_svd=int(bool(fast_svd)))
+                                       fast_svd=svd_mode(svd, fast_svd))
         self._cache = None
 
     def set_state_image(self, image) -> None:
@@ -330,7 +344,8 @@ class MPMSimulation(_Engine):
     def __init__(self, particles: Union[Particles, Sequence[Particle]], model: MaterialModel, res: int = 64,
                  dt: float = 1e-4, E: float = 1e4, nu: float = 0.2, gravity: float = -100.0, *,
                  identity_mode: int = 0, p2g_chunk: int = 0, g2p_threads: int = 0, graph: bool = False,
-                 autotune: bool = False, timing: bool = False, device: int = 0, fast_svd: bool = False):
+                 autotune: bool = False, timing: bool = False, device: int = 0, fast_svd: bool = False,
+                 svd: Optional[str] = None):
         if not isinstance(particles, Particles):
             particles = Particles.from_list(list(particles), identity_mode=identity_mode)
         if particles.dim not in (2, 3):
@@ -339,7 +354,7 @@ class MPMSimulation(_Engine):
         self._model = MaterialModel(model)
         self._nus = [float(nu)]
         self._init_common(particles.dim, particles.n, particles.c.copy(), identity_mode, p2g_chunk, g2p_threads,
-                          graph, autotune, timing, device, fast_svd)
+                          graph, autotune, timing, device, fast_svd, svd)
         h = ctypes.c_void_p()
         check(lib().mpm_create(self._dim, int(self._model), self._res, dt, E, nu, gravity, self._n,
                                ptr(particles.x), ptr(particles.v), ptr(particles.F), ptr(particles.C),
@@ -397,7 +412,7 @@ class MPMBatch(_Engine):
 
     def __init__(self, scenes: Sequence, *, identity_mode: int = 0, p2g_chunk: int = 0, g2p_threads: int = 0,
                  graph: bool = False, autotune: bool = False, timing: bool = False, device: int = 0,
-                 fast_svd: bool = False):
+                 fast_svd: bool = False, svd: Optional[str] = None):
         specs: List[SceneSpec] = []
         for s in scenes:
             if isinstance(s, SceneSpec):
@@ -414,7 +429,7 @@ class MPMBatch(_Engine):
         self._offsets = np.cumsum([0] + [sp.particles.n for sp in specs])
         allp = Particles.concatenate([sp.particles for sp in specs])
         self._init_common(allp.dim, allp.n, allp.c.copy(), identity_mode, p2g_chunk, g2p_threads, graph, autotune,
-                          timing, device, fast_svd)
+                          timing, device, fast_svd, svd)
         table = (mpm_scene * len(specs))()
         for i, sp in enumerate(specs):
             table[i] = mpm_scene(int(sp.material), int(sp.res), sp.dt, sp.E, sp.nu, sp.gravity, sp.particles.n)

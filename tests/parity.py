@@ -110,3 +110,20 @@ def grid_errors(gv, gm, ref_v, ref_m) -> dict:
     dv = np.abs(gv - ref_v).max(initial=0.0) / max(np.abs(ref_v).max(initial=0.0), 1e-30)
     dm = np.abs(gm - ref_m).max(initial=0.0) / max(np.abs(ref_m).max(initial=0.0), 1e-30)
     return {"grid_v": float(dv), "grid_m": float(dm)}
+
+
+def assert_accurate_parity(gpu, ref32, ref64, inv_dx: float, tol: float = TOL, label: str = "") -> dict:
+    """One-step agreement of a step whose polar factor and SVD are computed more accurately than
+    the reference's FP32 two-sided Jacobi (the engine's accurate SVD mode).  Per field f:
+        err(gpu, FP32 reference) <= tol,   or, where the reference's FP32 arithmetic itself is
+        farther than that from the FP64 step on f:  err(gpu, FP64) <= err(FP32 reference, FP64),
+    i.e. the engine is within tol of the reference, or at least as close to the exact step as the
+    reference's own FP32 result is.  Returns the three error dictionaries."""
+    e32 = field_errors(gpu, ref32, inv_dx)
+    e64 = field_errors(gpu, ref64, inv_dx)
+    r64 = field_errors(ref32, ref64, inv_dx)
+    b32, b64, br = bounded(e32), bounded(e64), bounded(r64)
+    bad = {k: (b32[k], b64[k], br[k]) for k in b32 if not (b32[k] <= tol or b64[k] <= br[k])}
+    assert not bad, (f"{label}: (vs FP32, vs FP64, FP32 reference vs FP64) above {tol} and farther from FP64 "
+                     f"than the reference: {bad}")
+    return {"vs_fp32": b32, "vs_fp64": b64, "fp32_vs_fp64": br}
