@@ -176,6 +176,14 @@ int mpm_active_nodes_scene(mpm_handle h, int scene, int64_t* count);
  * counterpart; the reference steps particles in index order, src/nclr.h:112,169). */
 int mpm_sort_keys(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, int device);
 
+/* mpm_sort_keys on device buffers without allocating: no counterpart in the reference, for callers
+ * that cannot allocate device memory (code under stream capture, serving loops that must not call
+ * hipMalloc).  keys (n, input, unchanged) and perm (n, output) are device memory on `device`; the
+ * sort is queued on `stream` (a hipStream_t; NULL: the null stream) and the call returns without
+ * waiting.  The temporary buffers live in a fixed device array of the library (about 12.9 MB per
+ * GPU, kept until the process ends): n is at most 2^20 (larger n: MPM_ERR_INVALID).  Calls that
+ * overlap on one GPU share that array and race: serialise them (one stream, or wait between). */
+int mpm_sort_keys_noalloc(const uint32_t* keys, int64_t n, int bits, uint32_t* perm, void* stream, int device);
 
 /* The step's device SVD on n host matrices (dim x dim, row-major): A = U diag(sigma) V^T with
  * nclr_svd's convention (src/nclr_math.h:50-74: descending order; in 3D U, V proper rotations and

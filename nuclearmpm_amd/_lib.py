@@ -134,6 +134,8 @@ def lib() -> ctypes.CDLL:
     L.mpm_active_nodes.restype = i32
     L.mpm_sort_keys.argtypes = [_vp, i64, i32, _vp, i32]
     L.mpm_sort_keys.restype = i32
+    L.mpm_sort_keys_noalloc.argtypes = [_vp, i64, i32, _vp, _vp, i32]
+    L.mpm_sort_keys_noalloc.restype = i32
     L.mpm_svd.argtypes = [i32, i64, _vp, _vp, _vp, _vp, i32, i32]
     L.mpm_svd.restype = i32
     L.mpm_grad_channels.argtypes = [i32]
