@@ -125,22 +125,31 @@ def cpu_baseline(sc, steps: int):
 
 def two_cubes_2d():
     """The reference example's 2D snow scene (src/example.cpp:29-46: two 50x50 cubes, 5,000
-    particles, res 64) for its 4,000 steps: tiny, so bounded by the per-step launch sequence
-    rather than by HBM.  The launch shape is the autotuner's, as for the headline workload."""
+    particles, res 64) for its 4,000 steps in one advance: tiny, so bounded by per-step latency
+    rather than by HBM.  The headline figure takes the autotuner's choice (the tiled step's launch
+    shapes and the compact step's particles per thread); the tiled step and each compact shape are
+    reported beside it."""
     from nuclearmpm_amd import MPMSimulation, scenes
 
     sc = scenes.two_cubes_2d()
-    with MPMSimulation(sc.particles, sc.material, res=sc.res, autotune=True) as sim:
-        sim.advance(20)
-        sim.synchronize()
-        t0 = time.perf_counter()
-        sim.advance(sc.steps)
-        sim.synchronize()
-        t = time.perf_counter() - t0
-        tuning = sim.tuning()
+
+    def timed(**kw):
+        with MPMSimulation(sc.particles, sc.material, res=sc.res, **kw) as sim:
+            sim.advance(20)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.advance(sc.steps)
+            sim.synchronize()
+            return time.perf_counter() - t0, sim.tuning()
+
+    t, tuning = timed(autotune=True)
+    variants = {"tiled": timed(compact=0)[0]}
+    for ppt in (1, 2, 4):
+        variants[f"compact_ppt{ppt}"] = timed(compact=ppt)[0]
     return {"workload": "two_cubes_2d: 2D snow, 2 x 50x50 particles, res 64, dt 1e-4 (src/example.cpp)",
             "particles": sc.particles.n, "steps": sc.steps, "value": sc.particles.n * sc.steps / t,
-            "unit": "particle-steps/s", "us_per_step": 1e6 * t / sc.steps, "tuning": tuning}
+            "unit": "particle-steps/s", "us_per_step": 1e6 * t / sc.steps, "tuning": tuning,
+            "us_per_step_by_step_kind": {k: 1e6 * v / sc.steps for k, v in variants.items()}}
 
 
 def batched_vs_sequential(steps: int):

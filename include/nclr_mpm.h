@@ -71,7 +71,10 @@ typedef struct mpm_options {
      * A^T A; closer to the FP64 step than the reference's FP32 arithmetic is, and faster than 0.
      * The adjoint kernels use mode 0's factors for modes 0 and 2. */
     int32_t fast_svd;
-    int32_t reserved[8];
+    /* Single-scene engines: the compact step with this many particles per thread (1, 2 or 4; 3D: 1
+     * or 2), 0 = off (the default).  See mpm_set_compact. */
+    int32_t compact;
+    int32_t reserved[7];
 } mpm_options;
 
 /* One independent scene of a batch: the reference ctor's arguments (src/nclr.h:74-75). */
@@ -162,8 +165,23 @@ int mpm_set_timing(mpm_handle h, int enable);
 /* Launch shape of the step kernels (engine extras, no reference counterpart). */
 int mpm_set_tuning(mpm_handle h, int p2g_chunk, int g2p_threads);
 int mpm_get_tuning(mpm_handle h, int* p2g_chunk, int* g2p_threads);
-/* Times `steps_per_trial` steps of every launch shape on the current state, keeps the fastest and
- * restores the state; returns the choice through mpm_get_tuning. */
+/* The compact step (engine extra, no reference counterpart; for small single-scene engines, where
+ * the tiled step's per-step launches and sort dominate): mpm_advance runs all its steps in one
+ * cooperative launch of ceil(n / (256 particles_per_thread)) workgroups, at most one per CU, with no
+ * sort: each workgroup deposits its particles into a fixed-point copy of the box of grid nodes the
+ * particles can touch (at most 4266 nodes, in LDS), the grid update sums the workgroups' copies in
+ * a fixed order, and G2P updates the particles in place.  Same step, same parity bound against the
+ * reference and deterministic, but not bitwise the tiled step (its fixed-point sums are grouped by
+ * workgroup, not by tile).  When the box outgrows the LDS the remaining steps run tiled (and the
+ * step stays tiled until the state is set again); mpm_advance then waits for the launch, which it
+ * does not when the whole grid fits (2D res <= 64, 3D res <= 15).  particles_per_thread: 1, 2 or 4
+ * (3D: 1 or 2), 0 = off.  MPM_ERR_UNSUPPORTED for batches and for more particles than one
+ * co-resident launch holds.  Off while timing is on. */
+int mpm_set_compact(mpm_handle h, int particles_per_thread);
+int mpm_get_compact(mpm_handle h, int* particles_per_thread);
+/* Times `steps_per_trial` steps of every launch shape (and, on single-scene engines, of the compact
+ * step) on the current state, keeps the fastest and restores the state; returns the choice through
+ * mpm_get_tuning and mpm_get_compact. */
 int mpm_autotune(mpm_handle h, int steps_per_trial);
 
 /* Number of grid nodes with mass > 0 after the last step's P2G in scene `scene` (-1: all scenes);

@@ -34,7 +34,8 @@ class mpm_options(ctypes.Structure):
         ("graph", ctypes.c_int32),
         ("autotune", ctypes.c_int32),
         ("fast_svd", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("compact", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -100,6 +101,10 @@ def lib() -> ctypes.CDLL:
     L.mpm_set_tuning.restype = i32
     L.mpm_get_tuning.argtypes = [H, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.mpm_get_tuning.restype = i32
+    L.mpm_set_compact.argtypes = [H, i32]
+    L.mpm_set_compact.restype = i32
+    L.mpm_get_compact.argtypes = [H, ctypes.POINTER(i32)]
+    L.mpm_get_compact.restype = i32
     L.mpm_autotune.argtypes = [H, i32]
     L.mpm_autotune.restype = i32
     L.mpm_active_nodes_scene.argtypes = [H, i32, ctypes.POINTER(i64)]

@@ -126,6 +126,14 @@ struct nclr_mpm_sim {
     char* sort_tape = nullptr;  // per recomputed step of a backward segment, its sort block
     int64_t tape_states = 0;
     int g2p_threads = 256;
+    // compact step (single scene; mpm_compact.hip): particles per thread, 0 = off; its buffers are
+    // allocated by the first compact launch
+    int compact_ppt = 0;
+    bool compact_overflowed = false;  // a launch's box outgrew the LDS since the state was last set
+    int compact_wg_limit = 0;         // co-resident workgroups of the compact step
+    float4 *cslots = nullptr, *cgrid = nullptr;
+    uint8_t* cmassive = nullptr;
+    int *cbox = nullptr, *cinfo = nullptr;
     int64_t steps = 0;
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     std::vector<PendingEvent> pending;
@@ -281,6 +289,101 @@ struct nclr_mpm_sim {
         return MPM_OK;
     }
 
+    // ---- the compact step --------------------------------------------------------------------
+    int compact_workgroups(int ppt) const { return int((p.n + 256 * int64_t(ppt) - 1) / (256 * int64_t(ppt))); }
+    bool compact_whole_fits() const {
+        int64_t nodes = 1;
+        for (int a = 0; a < dim; ++a) nodes *= scenes[0].res + 1;
+        return nodes <= nclr_host::compact_capacity(dim);
+    }
+    // co-resident workgroups of the compact step: one per CU (its LDS box takes 150 of the CU's
+    // 160 KiB), at most kCompactMaxWorkgroups
+    int compact_limit_checked(int& limit) {
+        if (!compact_wg_limit) {
+            int cus = 0;
+            HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, opts.device));
+            compact_wg_limit = std::max(1, std::min(cus, nclr_dev::kCompactMaxWorkgroups));
+        }
+        limit = compact_wg_limit;
+        return MPM_OK;
+    }
+    static bool valid_compact(int dim, int ppt) { return ppt == 0 || ppt == 1 || ppt == 2 || (dim == 2 && ppt == 4); }
+    int set_compact(int ppt) {
+        if (!valid_compact(dim, ppt))
+            return fail(MPM_ERR_INVALID, dim == 2 ? "compact must be 0, 1, 2 or 4" : "compact must be 0, 1 or 2 in 3D");
+        if (ppt && scenes.size() != 1) return fail(MPM_ERR_UNSUPPORTED, "the compact step runs single-scene engines");
+        if (ppt) {
+            int limit = 0;
+            RC_TRY(compact_limit_checked(limit));
+            if (compact_workgroups(ppt) > limit)
+                return fail(MPM_ERR_UNSUPPORTED, "the compact step with " + std::to_string(ppt) +
+                                                     " particles per thread holds at most " +
+                                                     std::to_string(int64_t(limit) * 256 * ppt) + " particles");
+        }
+        compact_ppt = ppt;
+        return MPM_OK;
+    }
+    bool compact_usable() const { return compact_ppt > 0 && !compact_overflowed && !opts.timing && p.n > 0; }
+    int ensure_compact() {
+        if (cslots) return MPM_OK;
+        const size_t cap = size_t(nclr_host::compact_capacity(dim));
+        HIP_TRY(hipStreamSynchronize(stream));
+        int limit = 0;
+        RC_TRY(compact_limit_checked(limit));
+        if (dmalloc(&cslots, size_t(limit) * cap) != hipSuccess || dmalloc(&cgrid, cap) != hipSuccess ||
+            dmalloc(&cmassive, cap) != hipSuccess || dmalloc(&cbox, size_t(4 * dim)) != hipSuccess ||
+            dmalloc(&cinfo, size_t(nclr_dev::kInfoWords)) != hipSuccess)
+            return fail(MPM_ERR_HIP, "compact step: hipMalloc failed");
+        return MPM_OK;
+    }
+    // Up to nsteps steps in one launch; `done` = the steps it took (fewer only when its box outgrew
+    // the LDS, which needs a read of the launch's info: when the whole grid fits, none is needed and
+    // the call does not wait).
+    int compact_steps(int nsteps, int& done) {
+        RC_TRY(ensure_compact());
+        nclr_dev::CompactArgs a{};
+        a.st = state[cur];
+        a.stride = p.stride;
+        a.n = p.n;
+        a.nsteps = nsteps;
+        a.sp = scenes[0];
+        a.step = p;
+        a.stamp = stamp;
+        a.slots = cslots;
+        a.grid = cgrid;
+        a.massive = cmassive;
+        a.box = cbox;
+        a.info = cinfo;
+        HIP_TRY(nclr_host::launch_compact_steps(dim, material, opts.fast_svd, compact_ppt,
+                                                compact_workgroups(compact_ppt), a, stream));
+        HIP_TRY(nclr_host::launch_compact_to_tiles(dim, a, gv, stamps, active, stream));
+        sorted_valid = false;  // particles moved many steps since the last sort
+        done = nsteps;
+        if (!compact_whole_fits()) {
+            int info[nclr_dev::kInfoWords];
+            HIP_TRY(hipMemcpyAsync(info, cinfo, sizeof(info), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            if (info[nclr_dev::kInfoReason] == nclr_dev::kCompactOverflow) {
+                done = info[nclr_dev::kInfoDone];
+                compact_overflowed = true;
+            }
+        }
+        steps += done;
+        stamp_host += done;
+        return MPM_OK;
+    }
+    // mpm_advance: the compact step while it is on and its box fits, the tiled step otherwise.
+    int advance(int nsteps) {
+        int left = nsteps;
+        if (left > 0 && compact_usable()) {
+            int done = 0;
+            RC_TRY(compact_steps(left, done));
+            left -= done;
+        }
+        for (int s = 0; s < left; ++s) RC_TRY(step());
+        return MPM_OK;
+    }
+
     // Waits for the stream, checks the engine-wide status word and reads the per-scene words;
     // the scenes' new status bits move into scene_seen (and are cleared on the device).
     int sync_read(std::vector<uint32_t>& flags, std::vector<uint32_t>& frozen) {
@@ -351,6 +454,7 @@ struct nclr_mpm_sim {
         HIP_TRY(nclr_host::launch_fill_u32(scene_flags, int64_t(scenes.size()), 0u, stream));
         HIP_TRY(nclr_host::launch_fill_u32(scene_frozen, int64_t(scenes.size()), 0u, stream));
         std::fill(scene_seen.begin(), scene_seen.end(), 0u);
+        compact_overflowed = false;
         return MPM_OK;
     }
 
@@ -460,29 +564,53 @@ struct nclr_mpm_sim {
             rc = fail(MPM_ERR_HIP, "autotune: state copy failed");
         if (rc == MPM_OK && hipMemcpyAsync(saved_pid, pid[cur], pid_bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
             rc = fail(MPM_ERR_HIP, "autotune: state copy failed");
-        int best_chunk = p.chunk, best_threads = g2p_threads;
+        int best_chunk = p.chunk, best_threads = g2p_threads, best_compact = 0;
         float best_ms = 3.0e38f;
         const int chunks[3] = {256, 512, 1024};
         const int threads[3] = {64, 128, 256};
+        // one trial: a warm-up step (and graph capture), then trial_steps timed steps
+        auto trial = [&](float& ms) -> int {
+            RC_TRY(advance(1));
+            HIP_TRY(hipEventRecord(e0, stream));
+            RC_TRY(advance(trial_steps));
+            HIP_TRY(hipEventRecord(e1, stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            return MPM_OK;
+        };
+        compact_ppt = 0;
         for (int ci = 0; ci < 3 && rc == MPM_OK; ++ci)
             for (int ti = 0; ti < 3 && rc == MPM_OK; ++ti) {
                 rc = restore();
                 if (rc == MPM_OK) rc = set_tuning(chunks[ci], threads[ti]);
-                if (rc == MPM_OK) rc = step();  // warm-up (and graph capture)
-                if (rc == MPM_OK && hipEventRecord(e0, stream) != hipSuccess) rc = fail(MPM_ERR_HIP, "event");
-                for (int s = 0; s < trial_steps && rc == MPM_OK; ++s) rc = step();
-                if (rc == MPM_OK && hipEventRecord(e1, stream) != hipSuccess) rc = fail(MPM_ERR_HIP, "event");
                 float ms = 0.0f;
-                if (rc == MPM_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                    rc = fail(MPM_ERR_HIP, "autotune: event timing failed");
+                if (rc == MPM_OK) rc = trial(ms);
                 if (rc == MPM_OK && ms < best_ms) {
                     best_ms = ms;
                     best_chunk = chunks[ci];
                     best_threads = threads[ti];
                 }
             }
+        // the compact step (single scenes small enough for one co-resident launch)
+        if (rc == MPM_OK) rc = set_tuning(best_chunk, best_threads);
+        for (int ppt : {1, 2, 4}) {
+            if (rc != MPM_OK || scenes.size() != 1 || !valid_compact(dim, ppt)) break;
+            int limit = 0;
+            rc = compact_limit_checked(limit);
+            if (rc != MPM_OK || compact_workgroups(ppt) > limit) continue;
+            rc = restore();
+            compact_ppt = ppt;
+            float ms = 0.0f;
+            if (rc == MPM_OK) rc = trial(ms);
+            if (rc == MPM_OK && ms < best_ms) {
+                best_ms = ms;
+                best_compact = ppt;
+            }
+            compact_ppt = 0;
+        }
         if (rc == MPM_OK) rc = restore();
         if (rc == MPM_OK) rc = set_tuning(best_chunk, best_threads);
+        if (rc == MPM_OK) compact_ppt = best_compact;
         // trial steps may have stopped scenes whose particles leave the grid; the real steps will
         // find them again
         if (rc == MPM_OK) rc = nclr_host::launch_fill_u32(status, 1, 0u, stream) == hipSuccess ? MPM_OK : fail(MPM_ERR_HIP, "status reset");
@@ -702,13 +830,16 @@ struct nclr_mpm_sim {
                         hist,     slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
                         gtmp[1],  tape,       sort_tape, uidx,     ucnt,        upref, ttotal, rtot,
-                        tile_scene, node_tile_scene, scene_flags, scene_frozen};
+                        tile_scene, node_tile_scene, scene_flags, scene_frozen, cslots, cgrid, cmassive, cbox, cinfo};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         uidx = ucnt = upref = ttotal = meta = rtot = nullptr;
         p.items = nullptr;
         tile_scene = node_tile_scene = nullptr;
         scene_flags = scene_frozen = nullptr;
+        cslots = cgrid = nullptr;
+        cmassive = nullptr;
+        cbox = cinfo = nullptr;
         p.tile_scene = p.node_tile_scene = nullptr;
         p.scene_flags = p.scene_frozen = nullptr;
         slots_b = gbar = nullptr;
@@ -893,6 +1024,10 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
             hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return cleanup(fail(MPM_ERR_HIP, "device initialisation failed"));
+    if (h->opts.compact) {
+        rc = h->set_compact(h->opts.compact);
+        if (rc != MPM_OK) return cleanup(rc);
+    }
     if (h->opts.autotune && n > 0) {
         rc = h->autotune(3);
         if (rc != MPM_OK) return cleanup(rc);
@@ -928,8 +1063,7 @@ int mpm_advance(mpm_handle h, int nsteps) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
     if (nsteps < 0) return fail(MPM_ERR_INVALID, "nsteps must be >= 0");
     HIP_TRY(hipSetDevice(h->opts.device));
-    for (int s = 0; s < nsteps; ++s) RC_TRY(h->step());
-    return MPM_OK;
+    return h->advance(nsteps);
 }
 
 int mpm_synchronize(mpm_handle h) {
@@ -1078,6 +1212,18 @@ int mpm_get_tuning(mpm_handle h, int* p2g_chunk, int* g2p_threads) {
     if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
     if (p2g_chunk) *p2g_chunk = h->p.chunk;
     if (g2p_threads) *g2p_threads = h->g2p_threads;
+    return MPM_OK;
+}
+
+int mpm_set_compact(mpm_handle h, int particles_per_thread) {
+    if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    return h->set_compact(particles_per_thread);
+}
+
+int mpm_get_compact(mpm_handle h, int* particles_per_thread) {
+    if (!h || !particles_per_thread) return fail(MPM_ERR_INVALID, "NULL argument");
+    *particles_per_thread = h->compact_ppt;
     return MPM_OK;
 }
 

@@ -79,6 +79,43 @@ struct StepParams {
     uint32_t* scene_frozen;
 };
 
+// The compact step (mpm_compact.hip): dynamic LDS per workgroup for its box of nodes.
+constexpr int kCompactLdsBytes = 150 * 1024;
+constexpr int kCompactMaxWorkgroups = 256;
+
+// Words of CompactArgs::info, written by the compact step at the end of its launch.
+enum CompactInfo : int {
+    kInfoDone = 0,     // steps completed by the launch
+    kInfoLo = 1,       // [3] origin of the box of the last completed step's grid
+    kInfoExt = 4,      // [3] its extent
+    kInfoNodes = 7,    // its nodes
+    kInfoReason = 8,   // why the launch ended: kCompactDone, kCompactOverflow or kCompactStopped
+    kInfoOutside = 9,  // raised by a particle outside the box (the step is retried over the whole grid)
+    kInfoStamp = 10,   // device stamp of the last completed step
+    kInfoWords = 16
+};
+enum CompactReason : int {
+    kCompactDone = 0,      // every step completed
+    kCompactOverflow = 1,  // the box outgrew the LDS: the remaining steps are left to the tiled step
+    kCompactStopped = 2    // a particle left the grid: the scene stopped (the stamp counts every step)
+};
+
+// Arguments of the compact step kernel (one scene, many steps per launch).
+struct CompactArgs {
+    float* st;             // SoA state, updated in place
+    int64_t stride;
+    int64_t n;
+    int nsteps;
+    SceneParams sp;
+    StepParams step;       // the scene's status and stop words (scene_flags, scene_frozen)
+    uint32_t* stamp;       // device step stamp: + 1 per completed step
+    float4* slots;         // workgroups x box capacity: each workgroup's box copy
+    float4* grid;          // box capacity: the updated grid of the box
+    uint8_t* massive;      // box capacity: 1 where the node's mass after P2G is > 0
+    int* box;              // 2 x [min[D], max[D]]: base ranges of the positions, by step parity
+    int* info;             // kInfoWords words (CompactInfo)
+};
+
 }  // namespace nclr_dev
 
 namespace nclr_host {
@@ -177,6 +214,15 @@ hipError_t launch_gyroid_scatter(const mpm_cube& box, float k, float t, float th
                                  int identity_mode, int64_t n, float* img, hipStream_t s);
 hipError_t launch_project(int64_t n, const float* pos, int centered, float* out, hipStream_t s);
 int64_t max_items(int64_t n, uint32_t ntiles);
+// The compact step: nodes of the largest box; one cooperative launch of nwg workgroups of 256
+// threads with ppt particles each running a.nsteps steps of a single-scene engine.
+int compact_capacity(int dim);
+hipError_t launch_compact_steps(int dim, int material, int mode, int ppt, int nwg, const nclr_dev::CompactArgs& a,
+                                hipStream_t s);
+// Writes the compact step's last grid into the tiled grid (gv, stamps, active) of a single-scene
+// engine, so that the grid export and active-node count read it as the tiled step's.
+hipError_t launch_compact_to_tiles(int dim, const nclr_dev::CompactArgs& a, float4* gv, uint32_t* stamps,
+                                   uint32_t* active, hipStream_t s);
 // Records msg as the calling thread's mpm_last_error() and returns code.
 int set_last_error(int code, const char* msg);
 int num_fields(int dim);

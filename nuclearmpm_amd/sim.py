@@ -171,13 +171,13 @@ class _Engine:
     _h = None
 
     def _init_common(self, dim: int, n: int, colors, identity_mode, p2g_chunk, g2p_threads, graph, autotune, timing,
-                     device, fast_svd=False, svd=None):
+                     device, fast_svd=False, svd=None, compact=0):
         self._dim = dim
         self._n = n
         self._colors = colors
         self.options = default_options(identity_mode=identity_mode, p2g_chunk=p2g_chunk, g2p_threads=g2p_threads,
                                        graph=int(graph), autotune=int(autotune), timing=int(timing), device=device,
-                                       fast_svd=svd_mode(svd, fast_svd))
+                                       fast_svd=svd_mode(svd, fast_svd), compact=int(compact))
         self._cache = None
 
     def set_state_image(self, image) -> None:
@@ -272,13 +272,24 @@ class _Engine:
         check(lib().mpm_set_tuning(self._handle(), int(p2g_chunk), int(g2p_threads)))
 
     def tuning(self):
+        """The launch shape: P2G work-item size, G2P workgroup size and the compact step's
+        particles per thread (0: the tiled step)."""
         c = ctypes.c_int()
         t = ctypes.c_int()
+        k = ctypes.c_int()
         check(lib().mpm_get_tuning(self._handle(), ctypes.byref(c), ctypes.byref(t)))
-        return {"p2g_chunk": c.value, "g2p_threads": t.value}
+        check(lib().mpm_get_compact(self._handle(), ctypes.byref(k)))
+        return {"p2g_chunk": c.value, "g2p_threads": t.value, "compact": k.value}
+
+    def set_compact(self, particles_per_thread: int) -> None:
+        """The compact step (single scenes): all steps of an advance in one cooperative launch of
+        ceil(n / (256 particles_per_thread)) workgroups, no sort, the grid box in LDS; 0 turns it
+        off.  Same parity bound as the tiled step, not bitwise equal to it."""
+        check(lib().mpm_set_compact(self._handle(), int(particles_per_thread)))
 
     def autotune(self, steps_per_trial: int = 3):
-        """Time every launch shape on the current state (restored afterwards) and keep the fastest."""
+        """Time every launch shape (and the compact step, on single scenes) on the current state
+        (restored afterwards) and keep the fastest."""
         check(lib().mpm_autotune(self._handle(), int(steps_per_trial)))
         return self.tuning()
 
@@ -345,7 +356,7 @@ class MPMSimulation(_Engine):
                  dt: float = 1e-4, E: float = 1e4, nu: float = 0.2, gravity: float = -100.0, *,
                  identity_mode: int = 0, p2g_chunk: int = 0, g2p_threads: int = 0, graph: bool = False,
                  autotune: bool = False, timing: bool = False, device: int = 0, fast_svd: bool = False,
-                 svd: Optional[str] = None):
+                 svd: Optional[str] = None, compact: int = 0):
         if not isinstance(particles, Particles):
             particles = Particles.from_list(list(particles), identity_mode=identity_mode)
         if particles.dim not in (2, 3):
@@ -354,7 +365,7 @@ class MPMSimulation(_Engine):
         self._model = MaterialModel(model)
         self._nus = [float(nu)]
         self._init_common(particles.dim, particles.n, particles.c.copy(), identity_mode, p2g_chunk, g2p_threads,
-                          graph, autotune, timing, device, fast_svd, svd)
+                          graph, autotune, timing, device, fast_svd, svd, compact)
         h = ctypes.c_void_p()
         check(lib().mpm_create(self._dim, int(self._model), self._res, dt, E, nu, gravity, self._n,
                                ptr(particles.x), ptr(particles.v), ptr(particles.F), ptr(particles.C),

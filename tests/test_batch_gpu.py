@@ -102,7 +102,7 @@ def test_every_launch_shape_meets_parity(chunk, threads):
         sim.advance(15)
         mid = sim.particles().copy()
     with MPMSimulation(mid, sc.material, res=64, p2g_chunk=chunk, g2p_threads=threads) as sim:
-        assert sim.tuning() == {"p2g_chunk": chunk, "g2p_threads": threads}
+        assert sim.tuning() == {"p2g_chunk": chunk, "g2p_threads": threads, "compact": 0}
         sim.advance(1)
         out = sim.particles()
     ref = O.run(to_oracle(mid), int(sc.material), res=64, steps=1)
