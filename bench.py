@@ -211,7 +211,9 @@ def fwd_bwd(which: str):
 
     if which == "jelly_2d_diff":
         specs = [scenes.jelly_2d_diff()]
-        sim = MPMSimulation(specs[0].particles, specs[0].material, res=specs[0].res, E=specs[0].E)
+        # the forward steps take the autotuner's step (the compact step, for this small scene); the
+        # adjoint's recomputed steps are tiled (the adjoint kernels walk the tile sort)
+        sim = MPMSimulation(specs[0].particles, specs[0].material, res=specs[0].res, E=specs[0].E, autotune=True)
         desc = "jelly_2d_diff: 140x140 particles at [0.3,0.5]^2, res 128, 512 steps, checkpoint every 64"
     else:
         specs = scenes.jelly_3d_batched(seed=2, scenes=8)
@@ -262,6 +264,7 @@ def fwd_bwd(which: str):
         gbs = nbytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         kernels[name] = {"avg_ms": avg_ms, "launches": launches, "alg_bytes": nbytes, "achieved_GBs": gbs,
                          "frac": gbs / HBM_PEAK_GBS}
+    tuning = sim.tuning()
     sim.close()
     return {
         "workload": desc,
@@ -274,6 +277,7 @@ def fwd_bwd(which: str):
         "grad_finite": bool(torch.isfinite(g).all()) and bool(torch.isfinite(ge).all()),
         "adjoint_kernels": kernels,
         "g_active_end": g_active,
+        "tuning": tuning,
     }
 
 

@@ -133,7 +133,8 @@ struct nclr_mpm_sim {
     int compact_wg_limit = 0;         // co-resident workgroups of the compact step
     float4 *cslots = nullptr, *cgrid = nullptr;
     uint8_t* cmassive = nullptr;
-    int *cbox = nullptr, *cinfo = nullptr;
+    int *cbox = nullptr, *cinfo = nullptr, *csubbox = nullptr;
+    uint32_t* cbar = nullptr;
     int64_t steps = 0;
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     std::vector<PendingEvent> pending;
@@ -330,9 +331,11 @@ struct nclr_mpm_sim {
         HIP_TRY(hipStreamSynchronize(stream));
         int limit = 0;
         RC_TRY(compact_limit_checked(limit));
-        if (dmalloc(&cslots, size_t(limit) * cap) != hipSuccess || dmalloc(&cgrid, cap) != hipSuccess ||
-            dmalloc(&cmassive, cap) != hipSuccess || dmalloc(&cbox, size_t(4 * dim)) != hipSuccess ||
-            dmalloc(&cinfo, size_t(nclr_dev::kInfoWords)) != hipSuccess)
+        if (dmalloc(&cslots, 2 * size_t(limit) * cap) != hipSuccess || dmalloc(&cgrid, cap) != hipSuccess ||
+            dmalloc(&cmassive, cap) != hipSuccess || dmalloc(&cbox, size_t(2 * dim)) != hipSuccess ||
+            dmalloc(&cinfo, size_t(nclr_dev::kInfoWords)) != hipSuccess ||
+            dmalloc(&csubbox, size_t(2 * nclr_dev::kCompactMaxWorkgroups * 6)) != hipSuccess ||
+            dmalloc(&cbar, size_t(2)) != hipSuccess)
             return fail(MPM_ERR_HIP, "compact step: hipMalloc failed");
         return MPM_OK;
     }
@@ -341,15 +344,23 @@ struct nclr_mpm_sim {
     // the call does not wait).
     int compact_steps(int nsteps, int& done) {
         RC_TRY(ensure_compact());
+        // slots in Hilbert order of the cells, so each workgroup's particles are neighbours
+        HIP_TRY(nclr_host::launch_compact_order(dim, state[cur], pid[cur], state[cur ^ 1], pid[cur ^ 1], p.n, p.stride,
+                                                nf(), scenes[0], keys, perm, keys_tmp, perm_tmp, hist, stream));
+        cur ^= 1;
         nclr_dev::CompactArgs a{};
         a.st = state[cur];
         a.stride = p.stride;
         a.n = p.n;
         a.nsteps = nsteps;
+        RC_TRY(compact_limit_checked(a.slot_sets));
         a.sp = scenes[0];
         a.step = p;
         a.stamp = stamp;
+        a.status = status;
         a.slots = cslots;
+        a.subbox = csubbox;
+        a.bar = cbar;
         a.grid = cgrid;
         a.massive = cmassive;
         a.box = cbox;
@@ -405,6 +416,9 @@ struct nclr_mpm_sim {
         if (any) HIP_TRY(nclr_host::launch_fill_u32(scene_flags, int64_t(ns), 0u, stream));
         if (st != 0) HIP_TRY(nclr_host::launch_fill_u32(status, 1, 0u, stream));
         if (any || st != 0) HIP_TRY(hipStreamSynchronize(stream));
+        if (st & nclr_dev::kStatusBarrierTimeout)
+            return fail(MPM_ERR_HIP, "the compact step's grid-wide barrier timed out (workgroups not co-resident?); "
+                                     "the state is undefined");
         if (st & nclr_dev::kStatusMovedTooFar)
             return fail(MPM_ERR_OUT_OF_RANGE,
                         "the tile sort placed a particle outside the particle range (the state was not produced "
@@ -830,7 +844,8 @@ struct nclr_mpm_sim {
                         hist,     slots, gv, stamps, status,
                         active,   active_sum, stamp,    d_scenes, slots_b, gbar, epart, erange, epartial, gtmp[0],
                         gtmp[1],  tape,       sort_tape, uidx,     ucnt,        upref, ttotal, rtot,
-                        tile_scene, node_tile_scene, scene_flags, scene_frozen, cslots, cgrid, cmassive, cbox, cinfo};
+                        tile_scene, node_tile_scene, scene_flags, scene_frozen, cslots, cgrid, cmassive, cbox, cinfo,
+                        csubbox, cbar};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         uidx = ucnt = upref = ttotal = meta = rtot = nullptr;
@@ -839,7 +854,8 @@ struct nclr_mpm_sim {
         scene_flags = scene_frozen = nullptr;
         cslots = cgrid = nullptr;
         cmassive = nullptr;
-        cbox = cinfo = nullptr;
+        cbox = cinfo = csubbox = nullptr;
+        cbar = nullptr;
         p.tile_scene = p.node_tile_scene = nullptr;
         p.scene_flags = p.scene_frozen = nullptr;
         slots_b = gbar = nullptr;
@@ -1029,7 +1045,7 @@ int create_impl(int dim, int nscenes, const mpm_scene* sc, const float* x, const
         if (rc != MPM_OK) return cleanup(rc);
     }
     if (h->opts.autotune && n > 0) {
-        rc = h->autotune(3);
+        rc = h->autotune(20);
         if (rc != MPM_OK) return cleanup(rc);
     }
     *out = h;

@@ -13,7 +13,8 @@ namespace nclr_dev {
 constexpr uint32_t kStatusOutOfRange = 1u;
 // Tiles per round of the tile scans (one workgroup of 1024 threads, 4 tiles per thread).
 constexpr uint32_t kScanRound = 4096;
-constexpr uint32_t kStatusMovedTooFar = 2u;  // a particle crossed more than one tile in a step
+constexpr uint32_t kStatusMovedTooFar = 2u;
+constexpr uint32_t kStatusBarrierTimeout = 4u;  // a grid-wide barrier of the compact step waited past its limit  // a particle crossed more than one tile in a step
 constexpr int kMixed = 3;  // material template value: per-scene material chosen at run time
 
 // Particle tiles: S^D cells (S = 16 in 2D, 8 in 3D).  A tile's P2G region covers the nodes its
@@ -106,13 +107,17 @@ struct CompactArgs {
     int64_t stride;
     int64_t n;
     int nsteps;
+    int slot_sets;         // workgroup slots per step parity (>= the launch's workgroups)
     SceneParams sp;
     StepParams step;       // the scene's status and stop words (scene_flags, scene_frozen)
     uint32_t* stamp;       // device step stamp: + 1 per completed step
-    float4* slots;         // workgroups x box capacity: each workgroup's box copy
-    float4* grid;          // box capacity: the updated grid of the box
-    uint8_t* massive;      // box capacity: 1 where the node's mass after P2G is > 0
-    int* box;              // 2 x [min[D], max[D]]: base ranges of the positions, by step parity
+    uint32_t* status;      // engine status word (kStatusBarrierTimeout)
+    float4* slots;         // 2 x slot_sets x box capacity: each workgroup's deposits, by step parity
+    int* subbox;           // 2 x kCompactMaxWorkgroups x [lo[3], hi[3]]: nodes each workgroup deposited to
+    float4* grid;          // box capacity: the updated grid of the box of the last completed step
+    uint8_t* massive;      // box capacity: 1 where that step's node mass after P2G is > 0
+    int* box;              // [min[D], max[D]]: base range of the positions before the first step
+    uint32_t* bar;         // [2]: grid barrier arrivals and generation
     int* info;             // kInfoWords words (CompactInfo)
 };
 
@@ -218,6 +223,13 @@ int64_t max_items(int64_t n, uint32_t ntiles);
 // threads with ppt particles each running a.nsteps steps of a single-scene engine.
 int compact_capacity(int dim);
 hipError_t launch_compact_steps(int dim, int material, int mode, int ppt, int nwg, const nclr_dev::CompactArgs& a,
+                                hipStream_t s);
+// The slot order of a compact launch: the state (every field) and the slot -> particle map
+// gathered into st_out / pid_out in Hilbert order of the particles' cells (keys, perm and the
+// temporaries are the tile sort's buffers).
+hipError_t launch_compact_order(int dim, const float* st, const uint32_t* pid, float* st_out, uint32_t* pid_out,
+                                int64_t n, int64_t stride, int fields, const nclr_dev::SceneParams& sp, uint32_t* keys,
+                                uint32_t* perm, uint32_t* keys_tmp, uint32_t* perm_tmp, uint32_t* hist,
                                 hipStream_t s);
 // Writes the compact step's last grid into the tiled grid (gv, stamps, active) of a single-scene
 // engine, so that the grid export and active-node count read it as the tiled step's.
