@@ -260,7 +260,10 @@ int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float*
 /* Device bytes of the tape mpm_backward_segment keeps for segments of `nsteps` steps (-1 for a
  * NULL handle or nsteps < 1): nsteps - 1 entries of one state snapshot (mpm_state_bytes) and one
  * sort (8 bytes per particle, 12 bytes per particle tile of every scene for its bounds and work
- * item offset, 16 bytes per possible work item, 256-byte aligned). */
+ * item offset, 16 bytes per possible work item, 256-byte aligned), and nsteps entries of the
+ * forward grid and its pre-update {momentum, mass} (32 bytes per node of every scene's node tiles;
+ * kept only while it leaves a quarter of the free device memory, else each adjoint step
+ * recomputes its grid). */
 int64_t mpm_tape_bytes(mpm_handle h, int nsteps);
 
 /* Sets Young's modulus and Poisson's ratio of one scene (mu_0, lambda_0 recomputed in float as

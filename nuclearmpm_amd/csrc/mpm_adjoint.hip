@@ -13,7 +13,8 @@
 //                 back through the material update: snow's F' = U clamp(S) V^T (the backward pass
 //                 of a function of the singular values) and Jp' = clamp(Jp det F^ / det F'),
 //                 liquid's F'_00 = prod S
-//   grid_adjoint  one workgroup per node tile: sums both slot sets, recomputes grid_op
+//   grid_adjoint  one workgroup per node tile: sums both slot sets (or, when the forward grid
+//                 kernel kept the pre-update sums, reads those and sums the adjoint slots), recomputes grid_op
 //                 (src/nclr.h:263-310) and maps the velocity cotangent to the cotangent of the
 //                 pre-update {momentum, mass} (clamped components and sticky nodes pass nothing)
 //   p2g_adjoint   per particle: gathers the grid cotangent through the P2G stencil and the fused
@@ -519,6 +520,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 template <int D>
 __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ slots,
                                                       const float4* __restrict__ slots_b,
+                                                      const float4* __restrict__ gpre,
                                                       const uint32_t* __restrict__ item_offset, StepParams p,
                                                       float4* __restrict__ gbar) {
     using TL = Tile<D>;
@@ -587,8 +589,26 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                 b.y += sb.y;
                 b.z += sb.z;
             };
+            auto add_b = [&](const float4 sb) {
+                b.x += sb.x;
+                b.y += sb.y;
+                b.z += sb.z;
+            };
             const int64_t q = int64_t(first[c]) * TL::REGION + r;
             int it = 0;
+            if (gpre) {  // the forward sums are kept: only the adjoint slots
+                for (; it + 4 <= count[c]; it += 4) {
+                    const int64_t s0 = q + int64_t(it) * TL::REGION;
+                    const float4 b0 = slots_b[s0], b1 = slots_b[s0 + TL::REGION], b2 = slots_b[s0 + 2 * TL::REGION],
+                                 b3 = slots_b[s0 + 3 * TL::REGION];
+                    add_b(b0);
+                    add_b(b1);
+                    add_b(b2);
+                    add_b(b3);
+                }
+                for (; it < count[c]; ++it) add_b(slots_b[q + int64_t(it) * TL::REGION]);
+                continue;
+            }
             for (; it + 2 <= count[c]; it += 2) {
                 const int64_t s0 = q + int64_t(it) * TL::REGION, s1 = s0 + TL::REGION;
                 const float4 a0 = slots[s0], b0 = slots_b[s0], a1 = slots[s1], b1 = slots_b[s1];
@@ -600,6 +620,7 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                 add(slots[s0], slots_b[s0]);
             }
         }
+        if (gpre) g = gpre[int64_t(nt_idx) * TL::NODES + l];  // the same sums, in the same order
         const float mom[3] = {g.x, g.y, g.z};
         const float vbar[3] = {b.x, b.y, b.z};
         const float m = g.w;
@@ -934,9 +955,9 @@ hipError_t launch_grid_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
     const unsigned tiles = a.p.node_tiles;
     if (tiles == 0 || a.p.n == 0) return hipSuccess;
     if (a.dim == 2)
-        k_grid_adjoint<2><<<tiles, 256, 0, a.stream>>>(a.slots, b.slots_b, a.item_offset, a.p, b.gbar);
+        k_grid_adjoint<2><<<tiles, 256, 0, a.stream>>>(a.slots, b.slots_b, a.gpre, a.item_offset, a.p, b.gbar);
     else
-        k_grid_adjoint<3><<<tiles, 256, 0, a.stream>>>(a.slots, b.slots_b, a.item_offset, a.p, b.gbar);
+        k_grid_adjoint<3><<<tiles, 256, 0, a.stream>>>(a.slots, b.slots_b, a.gpre, a.item_offset, a.p, b.gbar);
     return hipGetLastError();
 }
 

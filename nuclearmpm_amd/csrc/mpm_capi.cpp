@@ -125,6 +125,12 @@ struct nclr_mpm_sim {
     size_t sort_block_bytes = 0;
     char* sort_tape = nullptr;  // per recomputed step of a backward segment, its sort block
     int64_t tape_states = 0;
+    // per step of a backward segment, the forward grid (gv) and its pre-update sums (gpre): the
+    // adjoint steps then skip P2G and the grid update.  Dense (node tiles x NODES float4 each), so
+    // allocated only while device memory allows; without it each adjoint step recomputes them.
+    float4 *tape_gv = nullptr, *tape_gpre = nullptr;
+    int64_t tape_grid_states = 0;
+    float4* gpre = nullptr;  // the grid kernel's pre-update output (nullptr: not kept)
     int g2p_threads = 256;
     // compact step (single scene; mpm_compact.hip): particles per thread, 0 = off; its buffers are
     // allocated by the first compact launch
@@ -163,6 +169,7 @@ struct nclr_mpm_sim {
         a.item_offset = item_offset;
         a.slots = slots;
         a.gv = gv;
+        a.gpre = gpre;
         a.stamps = stamps;
         a.status = status;
         a.active = active;
@@ -713,14 +720,22 @@ struct nclr_mpm_sim {
     // One adjoint step of the current state: recompute sort, P2G and the grid update, then run the
     // adjoint kernels.  The state and the step counter are left unchanged.
     // presorted: the sort block already holds this state's sort (a backward segment restored it);
-    // only the device step stamp advances.
-    int backward_step(const float* gout, float* gin, double* grad_E, bool presorted = false) {
+    // only the device step stamp advances.  grid_kept: gv and gpre already hold this state's
+    // forward grid (a backward segment's recomputed step wrote them): only the adjoint kernels run.
+    int backward_step(const float* gout, float* gin, double* grad_E, bool presorted = false, bool grid_kept = false) {
         RC_TRY(ensure_adjoint());
         if (p.n == 0) return MPM_OK;
         const nclr_host::LaunchArgs a = args(cur);
         const nclr_host::AdjointArgs b{gout, gin, p.n, slots_b, gbar, epart, grad_E, erange, epartial, echunks};
         PendingEvent pe{};
         if (opts.timing) RC_TRY(record_start(pe, kBackward));
+        if (presorted && grid_kept) {
+            RC_TRY(timed(kG2PAdj, [&] { return nclr_host::launch_g2p_adjoint(a, b); }));
+            RC_TRY(timed(kGridAdj, [&] { return nclr_host::launch_grid_adjoint(a, b); }));
+            RC_TRY(timed(kP2GAdj, [&] { return nclr_host::launch_p2g_adjoint(a, b); }));
+            if (opts.timing) RC_TRY(record_stop(pe));
+            return MPM_OK;
+        }
         if (presorted) {
             HIP_TRY(nclr_host::launch_bump_u32(stamp, stream));
         } else {
@@ -745,8 +760,10 @@ struct nclr_mpm_sim {
     // laid out as save_state writes it, and state 0 stays in the engine's own buffer.  The steps
     // and the adjoint steps run on those buffers in place (the state pointers are redirected, no
     // copies).  Each recomputed step's sort block is kept in the sort tape and restored for that
-    // step's adjoint instead of sorting again.  The engine's own buffers, pointers and step counter
-    // are restored at the end, so the state is the checkpoint again.  Graph replay is off inside
+    // step's adjoint instead of sorting again, and, with the grid tape, each step's forward grid and
+    // pre-update sums are written straight into their tape entries, so the adjoint steps run only
+    // the adjoint kernels.  The engine's own buffers, pointers and step counter are restored at the
+    // end, so the state is the checkpoint again.  Graph replay is off inside
     // the segment (graphs hold the engine's own pointers).
     int backward_segment(int nsteps, const float* gout, float* gin, double* grad_E) {
         RC_TRY(ensure_adjoint());
@@ -763,6 +780,31 @@ struct nclr_mpm_sim {
             HIP_TRY(hipMalloc(reinterpret_cast<void**>(&sort_tape), sort_block_bytes * size_t(nsteps - 1)));
             tape_states = nsteps - 1;
         }
+        const size_t gv_nodes = size_t(p.node_tiles) * nclr_host::tile_nodes(dim);
+        if (tape_grid_states < nsteps) {
+            // the grid tape: every step of the segment (the last one's adjoint recomputes its grid
+            // into its entry), if it leaves a quarter of the free device memory
+            HIP_TRY(hipStreamSynchronize(stream));
+            if (tape_gv) (void)hipFree(tape_gv);
+            if (tape_gpre) (void)hipFree(tape_gpre);
+            tape_gv = tape_gpre = nullptr;
+            tape_grid_states = 0;
+            size_t free_b = 0, total_b = 0;
+            HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+            const size_t need = 2 * size_t(nsteps) * gv_nodes * sizeof(float4);
+            if (need <= free_b / 4 * 3 && dmalloc(&tape_gv, size_t(nsteps) * gv_nodes) == hipSuccess) {
+                if (dmalloc(&tape_gpre, size_t(nsteps) * gv_nodes) == hipSuccess) {
+                    tape_grid_states = nsteps;
+                } else {
+                    (void)hipFree(tape_gv);
+                    tape_gv = nullptr;
+                    (void)hipGetLastError();
+                }
+            } else {
+                (void)hipGetLastError();
+            }
+        }
+        const bool grid_tape = tape_grid_states >= nsteps;
         struct Restore {
             nclr_mpm_sim& e;
             float* state[2];
@@ -770,6 +812,7 @@ struct nclr_mpm_sim {
             int cur;
             int64_t steps;
             int graph;
+            float4* gv;
             ~Restore() {
                 for (int k = 0; k < 2; ++k) {
                     e.state[k] = state[k];
@@ -779,8 +822,15 @@ struct nclr_mpm_sim {
                 e.steps = steps;
                 e.opts.graph = graph;
                 e.sorted_valid = false;
+                e.gv = gv;
+                e.gpre = nullptr;
             }
-        } restore{*this, {state[0], state[1]}, {pid[0], pid[1]}, cur, steps, opts.graph};
+        } restore{*this, {state[0], state[1]}, {pid[0], pid[1]}, cur, steps, opts.graph, gv};
+        // the grid tape entry of step t; own: the engine's own grid (with the tape's pre-update sums)
+        auto grid_at = [&](int t, bool own) {
+            gv = grid_tape && !own ? tape_gv + size_t(t) * gv_nodes : restore.gv;
+            gpre = grid_tape ? tape_gpre + size_t(t) * gv_nodes : nullptr;
+        };
         opts.graph = 0;
         auto point_at = [&](int slot, int t) {  // slot <- state t of the segment
             if (t == 0) {
@@ -795,6 +845,7 @@ struct nclr_mpm_sim {
         // step t sorts state t: its sort block is kept for the adjoint of step t
         for (int t = 0; t + 1 < nsteps; ++t) {
             point_at(cur ^ 1, t + 1);
+            grid_at(t, false);
             RC_TRY(step());
             HIP_TRY(hipMemcpyAsync(sort_tape + sort_block_bytes * size_t(t), sort_block, sort_block_bytes,
                                    hipMemcpyDeviceToDevice, stream));
@@ -809,7 +860,10 @@ struct nclr_mpm_sim {
             if (kept)
                 HIP_TRY(hipMemcpyAsync(sort_block, sort_tape + sort_block_bytes * size_t(t), sort_block_bytes,
                                        hipMemcpyDeviceToDevice, stream));
-            RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E, kept));
+            // the segment's last adjoint step (t = 0) recomputes its grid into the engine's own grid,
+            // so the grid export and active-node count afterwards show a step from the checkpoint
+            grid_at(t, t == 0);
+            RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E, kept, kept && grid_tape && t > 0));
             c ^= 1;
         }
         HIP_TRY(hipMemcpyAsync(gin, gtmp[c], gbytes, hipMemcpyDeviceToDevice, stream));
@@ -865,6 +919,10 @@ struct nclr_mpm_sim {
         tape = nullptr;
         sort_block = sort_tape = nullptr;
         tape_states = 0;
+        if (tape_gv) (void)hipFree(tape_gv);
+        if (tape_gpre) (void)hipFree(tape_gpre);
+        tape_gv = tape_gpre = nullptr;
+        tape_grid_states = 0;
         state[0] = state[1] = nullptr;
         pid[0] = pid[1] = nullptr;
         keys = perm = keys_tmp = perm_tmp = hist = tile_start = tile_end = item_offset = nullptr;
@@ -1403,7 +1461,8 @@ int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float*
 
 int64_t mpm_tape_bytes(mpm_handle h, int nsteps) {
     if (!h || nsteps < 1) return -1;
-    return int64_t(nsteps - 1) * int64_t(h->state_bytes() + h->sort_block_bytes);
+    const int64_t grid = 2 * int64_t(h->p.node_tiles) * int64_t(nclr_host::tile_nodes(h->dim)) * int64_t(sizeof(float4));
+    return int64_t(nsteps - 1) * int64_t(h->state_bytes() + h->sort_block_bytes) + int64_t(nsteps) * grid;
 }
 
 int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu) {

@@ -147,6 +147,8 @@ struct LaunchArgs {
     uint32_t* item_offset;     // ntiles + 1: exclusive scan of P2G work items per tile
     float4* slots;             // max_items * REGION: the P2G region of every work item
     float4* gv;                // node tiles * NODES, updated grid
+    float4* gpre;              // optional (nullptr): node tiles * NODES, the grid's {momentum, mass} before
+                               // the update (written by the grid kernel, read by the grid adjoint)
     uint32_t* stamps;          // node tiles
     uint32_t* status;
     uint32_t* active;          // node tiles: nodes with post-P2G mass > 0

@@ -285,8 +285,8 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
 template <int D>
 __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ slots,
                                                     const uint32_t* __restrict__ item_offset, StepParams p,
-                                                    float4* __restrict__ gv, uint32_t* __restrict__ stamps,
-                                                    uint32_t* __restrict__ active) {
+                                                    float4* __restrict__ gv, float4* __restrict__ gpre,
+                                                    uint32_t* __restrict__ stamps, uint32_t* __restrict__ active) {
     using TL = Tile<D>;
     constexpr int NC = 1 << D;
     __shared__ int s_first[NC];  // first P2G work item of candidate tile c (its slot), or -1
@@ -384,6 +384,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
             }
         }
         massive_nodes += g.w > 0.0f ? 1u : 0u;
+        if (gpre) gpre[int64_t(nt_idx) * TL::NODES + l] = g;
         gv[int64_t(nt_idx) * TL::NODES + l] = grid_node<D>(g, n, sp);
     }
     // nodes with post-P2G mass > 0, per node tile (summed on demand by mpm_active_nodes)
@@ -654,9 +655,11 @@ hipError_t launch_grid(const LaunchArgs& a) {
     const unsigned tiles = a.p.node_tiles;
     if (tiles == 0) return hipSuccess;
     if (a.dim == 2)
-        k_grid_tiles<2><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+        k_grid_tiles<2><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.gpre, a.stamps,
+                                                                a.active);
     else
-        k_grid_tiles<3><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.stamps, a.active);
+        k_grid_tiles<3><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.gpre, a.stamps,
+                                                                a.active);
     return hipGetLastError();
 }
 
