@@ -248,8 +248,10 @@ NCLR_DI float ldg(const float* __restrict__ g, int64_t gstride, int c, int64_t j
 
 // ---------------------------------------------------------------------------------------------
 // G2P adjoint (src/nclr.h:167-258).  GENERAL: any material (the scene's, at run time); otherwise
-// every scene is jelly.  Register-limited to two waves per SIMD (at most 256 VGPRs; a few spills
-// in 3D): measured 1.45x faster than one wave per SIMD with ~270 VGPRs and no spills.
+// every scene is jelly.  Register-limited to two waves per SIMD (at most 256 VGPRs): measured 1.45x
+// faster than one wave per SIMD.  In 3D with two or more particles per thread, pass 1's results
+// (base, fx, the velocity and affine cotangents: 18 words a particle) wait for pass 2 in LDS
+// rather than in registers, which would spill.
 // ---------------------------------------------------------------------------------------------
 template <int D, int PPT, bool GENERAL, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
@@ -261,9 +263,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     using L = Layout<D>;
     constexpr int REG = TL::REGION;
     constexpr int K = D == 2 ? 9 : 27;
+    constexpr bool STASH = D == 3 && PPT > 1;
+    constexpr int NS = 3 * D + D * D;  // stashed words a particle
     __shared__ unsigned long long acc[D * REG];
     __shared__ uint32_t nonfinite[REG];
     __shared__ float s_bound[4][D];
+    __shared__ float s_stash[STASH ? PPT * NS * 256 : 1];
+    auto stash = [&](int k, int f) -> float& { return s_stash[(k * NS + f) * 256 + int(threadIdx.x)]; };
     const uint32_t item = blockIdx.x;
     const WorkItem wi = work_item(p, item);  // loaded with the count below, used only if valid
     if (item >= item_offset[p.ntiles]) return;
@@ -281,16 +287,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
     // pass 1: cotangents of the gathered velocity (vb = v' + dt x') and affine matrix
     // (Cb = C' + dt F' F^T) and a bound of the scattered sums per channel
+    constexpr int NR = STASH ? 1 : PPT;  // particles whose pass-1 results stay in registers
     bool valid[PPT];
-    int bases[PPT][D];
-    float fxs[PPT][D], vb[PPT][D];
-    M<D> Cb[PPT];
+    int bases[NR][D];
+    float fxs[NR][D], vb[NR][D];
+    M<D> Cb[NR];
     float bound[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) bound[c] = 0.0f;
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
+        const int kr = STASH ? 0 : k;
         valid[k] = false;
         if (j >= j1) continue;
         const int64_t i = perm[j];
@@ -309,9 +317,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         float wsum = 1.0f;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            bases[k][a] = base[a];
-            fxs[k][a] = fx[a];
-            vb[k][a] = ldg(gout, gstride, L::V + a, j) + sp.dt * ldg(gout, gstride, L::X + a, j);
+            bases[kr][a] = base[a];
+            fxs[kr][a] = fx[a];
+            vb[kr][a] = ldg(gout, gstride, L::V + a, j) + sp.dt * ldg(gout, gstride, L::X + a, j);
             wsum *= fabsf(w[a][0]) + fabsf(w[a][1]) + fabsf(w[a][2]);
         }
         // Fb: cotangent of F^ = (I' + dt C') F
@@ -379,16 +387,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 float s = 0.0f;
 #pragma unroll
                 for (int q = 0; q < D; ++q) s += Fb.a[a][q] * F.a[b][q];
-                Cb[k].a[a][b] = ldg(gout, gstride, L::C + a * D + b, j) + sp.dt * s;
+                Cb[kr].a[a][b] = ldg(gout, gstride, L::C + a * D + b, j) + sp.dt * s;
             }
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             float s = 0.0f;
 #pragma unroll
-            for (int b = 0; b < D; ++b) s += fabsf(Cb[k].a[a][b]);
+            for (int b = 0; b < D; ++b) s += fabsf(Cb[kr].a[a][b]);
             // |off - fx| <= 1.5 on every axis
-            const float bc = wsum * (fabsf(vb[k][a]) + 1.5f * four_inv_dx * s);
+            const float bc = wsum * (fabsf(vb[kr][a]) + 1.5f * four_inv_dx * s);
             if (bc < 3.0e38f) bound[a] = fmaxf(bound[a], bc);
+        }
+        if constexpr (STASH) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                stash(k, a) = __int_as_float(bases[0][a]);
+                stash(k, D + a) = fxs[0][a];
+                stash(k, 2 * D + a) = vb[0][a];
+#pragma unroll
+                for (int b = 0; b < D; ++b) stash(k, 3 * D + a * D + b) = Cb[0].a[a][b];
+            }
         }
     }
 #pragma unroll
@@ -415,13 +433,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
         const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
+        const int kr = STASH ? 0 : k;
+        if constexpr (STASH) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                bases[0][a] = __float_as_int(stash(k, a));
+                fxs[0][a] = stash(k, D + a);
+                vb[0][a] = stash(k, 2 * D + a);
+#pragma unroll
+                for (int b = 0; b < D; ++b) Cb[0].a[a][b] = stash(k, 3 * D + a * D + b);
+            }
+        }
         float w[D][3], dw[D][3];
         int rb = 0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            bspline_weights(fxs[k][a], w[a]);
-            bspline_dweights(fxs[k][a], dw[a]);
-            rb = rb * TL::R + (bases[k][a] - torg[a]);
+            bspline_weights(fxs[kr][a], w[a]);
+            bspline_dweights(fxs[kr][a], dw[a]);
+            rb = rb * TL::R + (bases[kr][a] - torg[a]);
         }
         M<D> Cn;
         float fxbar[D];
@@ -431,8 +460,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int b = 0; b < D; ++b) Cn.a[a][b] = 0.0f;
         }
-        const StencilRows<D> node(gv, bases[k], sp.ntn, sp.node_tile_base);
-#pragma unroll
+        const StencilRows<D> node(gv, bases[kr], sp.ntn, sp.node_tile_base);
+        // 3D: unrolled by rows of three nodes (fully unrolled, the early node loads spill)
+#pragma unroll(D == 3 ? 3 : K)
         for (int o = 0; o < K; ++o) {
             int off[D];
             offsets<D>(o, off);
@@ -440,14 +470,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             int ro = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                d[a] = float(off[a]) - fxs[k][a];
+                d[a] = float(off[a]) - fxs[kr][a];
                 ro = ro * TL::R + off[a];
             }
             const float4 g = node(off);
             const float gvel[3] = {g.x, g.y, g.z};
-            float weight = w[0][off[0]];
+            float weight = pick3(w[0], off[0]);
 #pragma unroll
-            for (int a = 1; a < D; ++a) weight *= w[a][off[a]];
+            for (int a = 1; a < D; ++a) weight *= pick3(w[a], off[a]);
             float wbar = 0.0f;
             const int r = rb + ro;
 #pragma unroll
@@ -455,11 +485,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 float cd = 0.0f;
 #pragma unroll
                 for (int b = 0; b < D; ++b) {
-                    cd += Cb[k].a[a][b] * d[b];
+                    cd += Cb[kr].a[a][b] * d[b];
                     Cn.a[a][b] += four_inv_dx * weight * gvel[a] * d[b];
                 }
-                const float val = weight * (vb[k][a] + four_inv_dx * cd);
-                wbar += gvel[a] * (vb[k][a] + four_inv_dx * cd);
+                const float val = weight * (vb[kr][a] + four_inv_dx * cd);
+                wbar += gvel[a] * (vb[kr][a] + four_inv_dx * cd);
                 const float y = val * scale[a];
                 if (fabsf(y) < kFixedLimit) atomicAdd(&acc[a * REG + r], fixed_point(y));
                 else atomicOr(&nonfinite[r], 1u << a);
@@ -468,15 +498,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int b = 0; b < D; ++b) {
                 float ctv = 0.0f;
 #pragma unroll
-                for (int a = 0; a < D; ++a) ctv += Cb[k].a[a][b] * gvel[a];
+                for (int a = 0; a < D; ++a) ctv += Cb[kr].a[a][b] * gvel[a];
                 fxbar[b] -= four_inv_dx * weight * ctv;
             }
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                float dwa = dw[a][off[a]];
+                float dwa = pick3(dw[a], off[a]);
 #pragma unroll
                 for (int b = 0; b < D; ++b)
-                    if (b != a) dwa *= w[b][off[b]];
+                    if (b != a) dwa *= pick3(w[b], off[b]);
                 fxbar[a] += wbar * dwa;
             }
         }
@@ -725,7 +755,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int b = 0; b < D; ++b) Abar.a[a][b] = 0.0f;
     }
     const StencilRows<D> node(gbar, base, sp.ntn, sp.node_tile_base);
-#pragma unroll
+    // 3D: unrolled by rows of three nodes (fully unrolled, the early node loads spill)
+#pragma unroll(D == 3 ? 3 : K)
     for (int o = 0; o < K; ++o) {
         int off[D];
         offsets<D>(o, off);
@@ -734,9 +765,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int a = 0; a < D; ++a) dpos[a] = (float(off[a]) - fx[a]) * sp.dx;
         const float4 g = node(off);
         const float gm[3] = {g.x, g.y, g.z};
-        float weight = w[0][off[0]];
+        float weight = pick3(w[0], off[0]);
 #pragma unroll
-        for (int a = 1; a < D; ++a) weight *= w[a][off[a]];
+        for (int a = 1; a < D; ++a) weight *= pick3(w[a], off[a]);
         float wbar = g.w * mass;
 #pragma unroll
         for (int r = 0; r < D; ++r) {
@@ -758,10 +789,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            float dwa = dw[a][off[a]];
+            float dwa = pick3(dw[a], off[a]);
 #pragma unroll
             for (int b = 0; b < D; ++b)
-                if (b != a) dwa *= w[b][off[b]];
+                if (b != a) dwa *= pick3(w[b], off[b]);
             fxbar[a] += wbar * dwa;
         }
     }

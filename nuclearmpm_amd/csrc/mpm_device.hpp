@@ -138,6 +138,13 @@ NCLR_DI int64_t gv_index(const int (&n)[D], int ntn, uint32_t node_tile_base) {
     return (int64_t(node_tile_base) + t) * TL::NODES + local;
 }
 
+// v[i] for i in {0, 1, 2} by selects: no private-array indexing (which would live in scratch)
+// when i is not a compile-time constant; folds to v[i] when it is.
+template <typename T>
+NCLR_DI T pick3(const T (&v)[3], int i) {
+    return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
 // The 3^D stencil nodes base + off of one particle in the updated grid, addressed per axis: node
 // n's byte offset inside its scene's node tiles is sum_a t_a ntn^(D-1-a) NODES 16 +
 // l_a S^(D-1-a) 16 with t_a = n_a >> LOG, l_a = n_a & (S-1) (gv_index's layout), so rows[a][o]
@@ -167,7 +174,7 @@ struct StencilRows {
     NCLR_DI float4 operator()(const int (&off)[D]) const {
         uint32_t b = 0;
 #pragma unroll
-        for (int a = 0; a < D; ++a) b += rows[a][off[a]];
+        for (int a = 0; a < D; ++a) b += pick3(rows[a], off[a]);
         return *reinterpret_cast<const float4*>(scene + b);
     }
 };

@@ -94,6 +94,7 @@ def test_backward_step_c_abi_and_permutations():
         check(L.mpm_load_state(h, snap.data_ptr()))
         gin = torch.empty_like(gu)
         gE = torch.zeros(1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()  # filled on torch's stream; the engine reads it on its own
         check(L.mpm_backward_step(h, gslot.data_ptr(), gin.data_ptr(), gE.data_ptr()))
         g0 = torch.empty_like(gu)
         check(L.mpm_grad_to_particle_order(h, gin.data_ptr(), g0.data_ptr()))
@@ -126,6 +127,7 @@ def test_backward_step_leaves_the_following_steps_unchanged():
         gout = torch.from_numpy(rng.normal(size=(ng, n)).astype(np.float32)).cuda()
         gin = torch.empty_like(gout)
         gE = torch.zeros(1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()  # filled on torch's stream; the engine reads it on its own
         check(L.mpm_backward_step(h, gout.data_ptr(), gin.data_ptr(), gE.data_ptr()))
         check(L.mpm_synchronize(h))
         for k in range(3):
@@ -437,20 +439,29 @@ def test_segment_with_the_grid_tape_equals_chained_single_steps(d):
             snaps.append(buf)
             sim.advance(1)
         gout = torch.from_numpy(rng.normal(size=(ng, n)).astype(np.float32)).cuda()
-        # chained single steps, from the last state back to the first
+        # chained single steps, from the last state back to the first (torch fills its tensors on
+        # its own stream: synchronise before the engine's stream reads them)
         g = gout.clone()
         gE_chain = torch.zeros(1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        # every buffer stays referenced until the engine's stream is done with it: torch's
+        # allocator would otherwise hand a released one to its own stream while the engine reads it
+        held = [g]
         for t in reversed(range(k)):
             check(L.mpm_load_state(h, snaps[t].data_ptr()))
             nxt = torch.empty_like(g)
+            held.append(nxt)
             check(L.mpm_backward_step(h, g.data_ptr(), nxt.data_ptr(), gE_chain.data_ptr()))
             g = nxt
+        check(L.mpm_synchronize(h))
         # one segment from the first state
         check(L.mpm_load_state(h, snaps[0].data_ptr()))
         gseg = gout.clone()
         gE_seg = torch.zeros(1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
         check(L.mpm_backward_segment(h, k, gseg.data_ptr(), gseg.data_ptr(), gE_seg.data_ptr()))
         check(L.mpm_synchronize(h))
         print(f"[adjoint] {d}D tape bytes for {k} steps: {int(L.mpm_tape_bytes(h, k))}", flush=True)
-    assert torch.equal(gseg, g)
+    diff = float((gseg - g).abs().max())
+    assert torch.equal(gseg, g), (diff, float(g.abs().max()), float(gE_seg), float(gE_chain))
     assert float(gE_seg) == float(gE_chain)
