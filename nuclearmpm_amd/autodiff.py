@@ -98,7 +98,8 @@ class _Rollout(torch.autograd.Function):
         sim = ctx.sim
         L, h = lib(), sim._handle()
         g_user = grad_out.contiguous().float()
-        _ready(g_user)
+        gE = torch.zeros(sim.scenes, dtype=torch.float64, device=g_user.device)
+        _ready(g_user)  # also completes gE's fill: the engine accumulates into it on its own stream
         # the segment adjoint recomputes the forward with the engine's current elasticity: restore
         # this rollout's E (another rollout or set_elasticity may have changed it since)
         for s, (e, nu) in enumerate(zip(ctx.E_vals, ctx.nu_vals)):
@@ -106,7 +107,6 @@ class _Rollout(torch.autograd.Function):
         g = torch.empty_like(g_user)
         check(L.mpm_load_state(h, ctx.final.data_ptr()))
         check(L.mpm_grad_from_particle_order(h, g_user.data_ptr(), g.data_ptr()))
-        gE = torch.zeros(sim.scenes, dtype=torch.float64, device=g.device)
         for buf, k in reversed(ctx.ckpts):
             check(L.mpm_load_state(h, buf.data_ptr()))
             check(L.mpm_backward_segment(h, int(k), g.data_ptr(), g.data_ptr(), gE.data_ptr()))
