@@ -248,10 +248,8 @@ NCLR_DI float ldg(const float* __restrict__ g, int64_t gstride, int c, int64_t j
 
 // ---------------------------------------------------------------------------------------------
 // G2P adjoint (src/nclr.h:167-258).  GENERAL: any material (the scene's, at run time); otherwise
-// every scene is jelly.  Register-limited to two waves per SIMD (at most 256 VGPRs): measured 1.45x
-// faster than one wave per SIMD.  In 3D with two or more particles per thread, pass 1's results
-// (base, fx, the velocity and affine cotangents: 18 words a particle) wait for pass 2 in LDS
-// rather than in registers, which would spill.
+// every scene is jelly.  At least two waves per SIMD (at most 256 VGPRs): measured 1.45x faster
+// than one wave per SIMD.
 // ---------------------------------------------------------------------------------------------
 template <int D, int PPT, bool GENERAL, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
@@ -263,13 +261,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     using L = Layout<D>;
     constexpr int REG = TL::REGION;
     constexpr int K = D == 2 ? 9 : 27;
-    constexpr bool STASH = D == 3 && PPT > 1;
-    constexpr int NS = 3 * D + D * D;  // stashed words a particle
     __shared__ unsigned long long acc[D * REG];
     __shared__ uint32_t nonfinite[REG];
     __shared__ float s_bound[4][D];
-    __shared__ float s_stash[STASH ? PPT * NS * 256 : 1];
-    auto stash = [&](int k, int f) -> float& { return s_stash[(k * NS + f) * 256 + int(threadIdx.x)]; };
     const uint32_t item = blockIdx.x;
     const WorkItem wi = work_item(p, item);  // loaded with the count below, used only if valid
     if (item >= item_offset[p.ntiles]) return;
@@ -287,18 +281,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
     // pass 1: cotangents of the gathered velocity (vb = v' + dt x') and affine matrix
     // (Cb = C' + dt F' F^T) and a bound of the scattered sums per channel
-    constexpr int NR = STASH ? 1 : PPT;  // particles whose pass-1 results stay in registers
     bool valid[PPT];
-    int bases[NR][D];
-    float fxs[NR][D], vb[NR][D];
-    M<D> Cb[NR];
+    int bases[PPT][D];
+    float fxs[PPT][D], vb[PPT][D];
+    M<D> Cb[PPT];
     float bound[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) bound[c] = 0.0f;
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
-        const int kr = STASH ? 0 : k;
         valid[k] = false;
         if (j >= j1) continue;
         const int64_t i = perm[j];
@@ -317,9 +309,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         float wsum = 1.0f;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            bases[kr][a] = base[a];
-            fxs[kr][a] = fx[a];
-            vb[kr][a] = ldg(gout, gstride, L::V + a, j) + sp.dt * ldg(gout, gstride, L::X + a, j);
+            bases[k][a] = base[a];
+            fxs[k][a] = fx[a];
+            vb[k][a] = ldg(gout, gstride, L::V + a, j) + sp.dt * ldg(gout, gstride, L::X + a, j);
             wsum *= fabsf(w[a][0]) + fabsf(w[a][1]) + fabsf(w[a][2]);
         }
         // Fb: cotangent of F^ = (I' + dt C') F
@@ -387,26 +379,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 float s = 0.0f;
 #pragma unroll
                 for (int q = 0; q < D; ++q) s += Fb.a[a][q] * F.a[b][q];
-                Cb[kr].a[a][b] = ldg(gout, gstride, L::C + a * D + b, j) + sp.dt * s;
+                Cb[k].a[a][b] = ldg(gout, gstride, L::C + a * D + b, j) + sp.dt * s;
             }
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             float s = 0.0f;
 #pragma unroll
-            for (int b = 0; b < D; ++b) s += fabsf(Cb[kr].a[a][b]);
+            for (int b = 0; b < D; ++b) s += fabsf(Cb[k].a[a][b]);
             // |off - fx| <= 1.5 on every axis
-            const float bc = wsum * (fabsf(vb[kr][a]) + 1.5f * four_inv_dx * s);
+            const float bc = wsum * (fabsf(vb[k][a]) + 1.5f * four_inv_dx * s);
             if (bc < 3.0e38f) bound[a] = fmaxf(bound[a], bc);
-        }
-        if constexpr (STASH) {
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                stash(k, a) = __int_as_float(bases[0][a]);
-                stash(k, D + a) = fxs[0][a];
-                stash(k, 2 * D + a) = vb[0][a];
-#pragma unroll
-                for (int b = 0; b < D; ++b) stash(k, 3 * D + a * D + b) = Cb[0].a[a][b];
-            }
         }
     }
 #pragma unroll
@@ -433,24 +415,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
         const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
-        const int kr = STASH ? 0 : k;
-        if constexpr (STASH) {
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                bases[0][a] = __float_as_int(stash(k, a));
-                fxs[0][a] = stash(k, D + a);
-                vb[0][a] = stash(k, 2 * D + a);
-#pragma unroll
-                for (int b = 0; b < D; ++b) Cb[0].a[a][b] = stash(k, 3 * D + a * D + b);
-            }
-        }
         float w[D][3], dw[D][3];
         int rb = 0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            bspline_weights(fxs[kr][a], w[a]);
-            bspline_dweights(fxs[kr][a], dw[a]);
-            rb = rb * TL::R + (bases[kr][a] - torg[a]);
+            bspline_weights(fxs[k][a], w[a]);
+            bspline_dweights(fxs[k][a], dw[a]);
+            rb = rb * TL::R + (bases[k][a] - torg[a]);
         }
         M<D> Cn;
         float fxbar[D];
@@ -460,7 +431,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int b = 0; b < D; ++b) Cn.a[a][b] = 0.0f;
         }
-        const StencilRows<D> node(gv, bases[kr], sp.ntn, sp.node_tile_base);
+        const StencilRows<D> node(gv, bases[k], sp.ntn, sp.node_tile_base);
         // 3D: unrolled by rows of three nodes (fully unrolled, the early node loads spill)
 #pragma unroll(D == 3 ? 3 : K)
         for (int o = 0; o < K; ++o) {
@@ -470,7 +441,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             int ro = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                d[a] = float(off[a]) - fxs[kr][a];
+                d[a] = float(off[a]) - fxs[k][a];
                 ro = ro * TL::R + off[a];
             }
             const float4 g = node(off);
@@ -485,11 +456,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 float cd = 0.0f;
 #pragma unroll
                 for (int b = 0; b < D; ++b) {
-                    cd += Cb[kr].a[a][b] * d[b];
+                    cd += Cb[k].a[a][b] * d[b];
                     Cn.a[a][b] += four_inv_dx * weight * gvel[a] * d[b];
                 }
-                const float val = weight * (vb[kr][a] + four_inv_dx * cd);
-                wbar += gvel[a] * (vb[kr][a] + four_inv_dx * cd);
+                const float val = weight * (vb[k][a] + four_inv_dx * cd);
+                wbar += gvel[a] * (vb[k][a] + four_inv_dx * cd);
                 const float y = val * scale[a];
                 if (fabsf(y) < kFixedLimit) atomicAdd(&acc[a * REG + r], fixed_point(y));
                 else atomicOr(&nonfinite[r], 1u << a);
@@ -498,7 +469,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int b = 0; b < D; ++b) {
                 float ctv = 0.0f;
 #pragma unroll
-                for (int a = 0; a < D; ++a) ctv += Cb[kr].a[a][b] * gvel[a];
+                for (int a = 0; a < D; ++a) ctv += Cb[k].a[a][b] * gvel[a];
                 fxbar[b] -= four_inv_dx * weight * ctv;
             }
 #pragma unroll
