@@ -57,7 +57,8 @@ typedef struct mpm_options {
     int32_t g2p_threads;
     /* 1: launch each step as one replay of a captured hipGraph (ignored while timing is on). */
     int32_t graph;
-    /* 1: choose p2g_chunk and g2p_threads at creation by timing trial steps (state restored). */
+    /* 1: choose p2g_chunk, g2p_threads and (single-scene engines) the compact step at creation by
+     * timing 20 trial steps of each (state restored); see mpm_autotune. */
     int32_t autotune;
     /* 3D singular value decompositions (the polar factor in P2G, the snow / liquid SVD in G2P):
      * 0 = nclr_svd's two-sided Jacobi in Eigen's operation order (src/nclr_math.h:50-74), bitwise
