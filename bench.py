@@ -320,12 +320,92 @@ def scene_generation():
 
 
 LEGS = {
+    "four_cubes_3d_accurate": lambda: accurate_step(),
     "two_cubes_2d": lambda: two_cubes_2d(),
     "batched": lambda: batched_vs_sequential(2000),
     "fwd_bwd_2d": lambda: fwd_bwd("jelly_2d_diff"),
     "fwd_bwd_3d": lambda: fwd_bwd("jelly_3d_batched"),
     "scene_gen": lambda: scene_generation(),
 }
+
+
+SVD_MODES = {"exact": "nclr_svd order (bitwise the reference's FP32)",
+             "fast": "fast FP32 Jacobi of A^T A (not bitwise the reference)",
+             "accurate": "FP64 Newton polar factor and FP64 Jacobi SVD (closer to the FP64 step than the "
+                         "reference's FP32 arithmetic)"}
+
+
+def measure_step(sc, svd: str, steps: int, warmup: int, tuning_arg: str = "auto", graph: bool = False):
+    """The plain step on scene `sc`: `warmup` untimed steps, the launch shape (autotuned on the
+    warmed-up state or given as CHUNKxTHREADS), `steps` timed steps, then a separate pass with HIP
+    events per kernel (so the events do not perturb the end-to-end number).  Returns (particle-steps/s,
+    ms per step, tuning, per-kernel records, nodes with mass, whether every position is finite)."""
+    from nuclearmpm_amd import MPMSimulation
+
+    p = sc.particles
+    n = p.n
+    dim, mat = p.dim, int(sc.material)
+    sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity,
+                        graph=graph, svd=svd)
+    sim.advance(warmup)
+    sim.synchronize()
+    if tuning_arg == "auto":
+        tuning = sim.autotune(5)  # untimed; restores the state and the step count
+    else:
+        chunk, threads = (int(v) for v in tuning_arg.lower().split("x"))
+        sim.set_tuning(chunk, threads)
+        tuning = sim.tuning()
+    sim.advance(2)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    total = t1 - t0
+    ms_per_step = 1e3 * total / steps
+    value = n * steps / total
+
+    ksteps = max(10, min(steps, 200))
+    sim.set_timing(True)
+    sim.reset_kernel_times()
+    sim.advance(ksteps)
+    sim.synchronize()
+    kt = {k: sim.kernel_time(k)[0] for k in ("p2g", "grid", "g2p")}
+    sim.set_timing(False)
+    g_active = sim.active_nodes()
+    finite = bool(__import__("numpy").isfinite(sim.particles().x).all())
+    sim.close()
+
+    p2g_bytes = P2G_BYTES_PER_PARTICLE[(dim, mat)] * n + P2G_BYTES_PER_NODE[dim] * g_active
+    g2p_bytes = G2P_BYTES_PER_PARTICLE[(dim, mat)] * n + G2P_BYTES_PER_NODE[dim] * g_active
+    grid_bytes = GRID_BYTES_PER_NODE[dim] * g_active
+    per_kernel = {
+        "p2g": {"avg_ms": kt["p2g"], "alg_bytes": p2g_bytes},
+        "grid": {"avg_ms": kt["grid"], "alg_bytes": grid_bytes},
+        "g2p": {"avg_ms": kt["g2p"], "alg_bytes": g2p_bytes},
+    }
+    # P2G accumulates every particle's 3^d x (d + 1) quantised contributions with 64-bit integer
+    # atomics in LDS (no HBM atomics: each work item stores its own slot)
+    lds_atomics = (3 ** dim) * (dim + 1) * n
+    per_kernel["p2g"]["lds_atomic_u64_per_launch"] = lds_atomics
+    per_kernel["p2g"]["lds_atomics_per_s"] = lds_atomics / (kt["p2g"] * 1e-3) if kt["p2g"] > 0 else 0.0
+    for v in per_kernel.values():
+        v["achieved_GBs"] = v["alg_bytes"] / (v["avg_ms"] * 1e-3) / 1e9 if v["avg_ms"] > 0 else 0.0
+        v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
+    return value, ms_per_step, tuning, per_kernel, g_active, finite
+
+
+def accurate_step():
+    """The headline workload (1M-particle 3D snow, 1000 timed steps after 20 warm-up steps,
+    autotuned) in the accurate SVD mode: per-particle results closer to the FP64 step than the
+    reference's FP32 arithmetic, not bitwise the reference's SVD."""
+    from nuclearmpm_amd import scenes
+
+    value, ms, tuning, per_kernel, g_active, finite = measure_step(scenes.four_cubes_3d(), "accurate", 1000, 20)
+    return {"workload": "four_cubes_3d, svd accurate", "value": value, "unit": "particle-steps/s",
+            "ms_per_step": ms, "svd": SVD_MODES["accurate"],
+            "tuning": {"p2g_chunk": int(tuning["p2g_chunk"]), "g2p_threads": int(tuning["g2p_threads"])},
+            "kernels": per_kernel, "g_active": g_active, "state_finite": finite}
 
 
 def main() -> int:
@@ -361,12 +441,11 @@ def main() -> int:
         print(json.dumps({args.leg: LEGS[args.leg]()}), flush=True)
         return 0
 
-    from nuclearmpm_amd import MPMSimulation, scenes
+    from nuclearmpm_amd import scenes
 
     sc = scenes.four_cubes_3d()
-    p = sc.particles
-    n = p.n
-    dim, mat = sc.particles.dim, int(sc.material)
+    n = sc.particles.n
+    dim = sc.particles.dim
 
     if args.copy_calibration:
         import torch
@@ -379,55 +458,8 @@ def main() -> int:
         del src, dst
 
     svd = args.svd or ("fast" if args.fast_svd else "exact")
-    sim = MPMSimulation(p, sc.material, res=sc.res, dt=sc.dt, E=sc.E, nu=sc.nu, gravity=sc.gravity,
-                        graph=args.graph, svd=svd)
-    sim.advance(args.warmup)
-    sim.synchronize()
-    if args.tuning == "auto":
-        tuning = sim.autotune(5)  # untimed; restores the state and the step count
-    else:
-        chunk, threads = (int(v) for v in args.tuning.lower().split("x"))
-        sim.set_tuning(chunk, threads)
-        tuning = sim.tuning()
-    sim.advance(2)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.advance(args.steps)
-    sim.synchronize()
-    t1 = time.perf_counter()
-    total = t1 - t0
-    ms_per_step = 1e3 * total / args.steps
-    value = n * args.steps / total
-
-    # per-kernel device times with HIP events on the engine stream (a separate pass, so the events
-    # do not perturb the end-to-end number above)
-    ksteps = max(10, min(args.steps, 200))
-    sim.set_timing(True)
-    sim.reset_kernel_times()
-    sim.advance(ksteps)
-    sim.synchronize()
-    kt = {k: sim.kernel_time(k)[0] for k in ("p2g", "grid", "g2p")}
-    sim.set_timing(False)
-    g_active = sim.active_nodes()
-    finite = bool(__import__("numpy").isfinite(sim.particles().x).all())
-    sim.close()
-
-    p2g_bytes = P2G_BYTES_PER_PARTICLE[(dim, mat)] * n + P2G_BYTES_PER_NODE[dim] * g_active
-    g2p_bytes = G2P_BYTES_PER_PARTICLE[(dim, mat)] * n + G2P_BYTES_PER_NODE[dim] * g_active
-    grid_bytes = GRID_BYTES_PER_NODE[dim] * g_active
-    per_kernel = {
-        "p2g": {"avg_ms": kt["p2g"], "alg_bytes": p2g_bytes},
-        "grid": {"avg_ms": kt["grid"], "alg_bytes": grid_bytes},
-        "g2p": {"avg_ms": kt["g2p"], "alg_bytes": g2p_bytes},
-    }
-    # P2G accumulates every particle's 3^d x (d + 1) quantised contributions with 64-bit integer
-    # atomics in LDS (no HBM atomics: each work item stores its own slot)
-    lds_atomics = (3 ** dim) * (dim + 1) * n
-    per_kernel["p2g"]["lds_atomic_u64_per_launch"] = lds_atomics
-    per_kernel["p2g"]["lds_atomics_per_s"] = lds_atomics / (kt["p2g"] * 1e-3) if kt["p2g"] > 0 else 0.0
-    for v in per_kernel.values():
-        v["achieved_GBs"] = v["alg_bytes"] / (v["avg_ms"] * 1e-3) / 1e9 if v["avg_ms"] > 0 else 0.0
-        v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
+    value, ms_per_step, tuning, per_kernel, g_active, finite = measure_step(
+        sc, svd, args.steps, args.warmup, args.tuning, args.graph)
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
     achieved = per_kernel[dom]["achieved_GBs"]
     traffic, traffic_state = load_traffic(dom)
@@ -456,10 +488,7 @@ def main() -> int:
             "hipgraph": args.graph,
             "tuning": {"p2g_chunk": int(tuning["p2g_chunk"]), "g2p_threads": int(tuning["g2p_threads"]),
                        "chosen_by": "autotuner" if args.tuning == "auto" else "flag"},
-            "svd": {"exact": "nclr_svd order (bitwise the reference's FP32)",
-                    "fast": "fast FP32 Jacobi of A^T A (not bitwise the reference)",
-                    "accurate": "FP64 Newton polar factor and FP64 Jacobi SVD (closer to the FP64 step than the "
-                                "reference's FP32 arithmetic)"}[svd],
+            "svd": SVD_MODES[svd],
         },
         "roofline": {
             "kernel": dom,

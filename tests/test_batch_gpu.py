@@ -5,11 +5,13 @@ step is deterministic: a scene advanced inside a batch must equal the same scene
 bit for bit, a graph replay must equal the eager launch sequence bit for bit, and every launch
 shape must meet the one-step parity bound against the oracle.
 """
+import functools
+
 import numpy as np
 import pytest
 
 import oracle as O
-from parity import assert_parity
+from parity import assert_parity, assert_parity_with_floor, reorder_noise
 
 from nuclearmpm_amd import MaterialModel, MPMBatch, MPMSimulation, Particles, SceneSpec, scenes
 
@@ -94,20 +96,42 @@ def test_graph_replays_equal_eager_with_interleaved_work():
         assert_same(g.particles(), eager.particles(), "after set_particles")
 
 
-@pytest.mark.parametrize("chunk", [256, 512, 1024])
-@pytest.mark.parametrize("threads", [64, 256])
-def test_every_launch_shape_meets_parity(chunk, threads):
+@functools.lru_cache(maxsize=None)
+def _launch_shape_case(identity_mode):
+    """Four 24^3 snow cubes after 15 steps at res 64, the oracle's step from there and the oracle's
+    own reorder noise on that state.  In the reference identity mode (F0 = diag(1, 1, 0)) the snow
+    sits on its clamp and the step is ill-conditioned in F; the physical identity mode is not."""
     sc = scenes.four_cubes_3d(side_res=24)
-    with MPMSimulation(sc.particles, sc.material, res=64) as sim:
+    p = sc.particles if identity_mode == 0 else Particles.from_positions(sc.particles.x, v=sc.particles.v,
+                                                                          identity_mode=1)
+    with MPMSimulation(p, sc.material, res=64, identity_mode=identity_mode) as sim:
         sim.advance(15)
         mid = sim.particles().copy()
-    with MPMSimulation(mid, sc.material, res=64, p2g_chunk=chunk, g2p_threads=threads) as sim:
+    ref, floor = reorder_noise(to_oracle(mid), lambda st: O.run(st, int(sc.material), res=64, steps=1,
+                                                                identity_mode=identity_mode), 64.0)
+    return sc, mid, ref, floor
+
+
+@pytest.mark.parametrize("identity_mode", [0, 1], ids=["reference_identity", "physical_identity"])
+@pytest.mark.parametrize("chunk", [256, 512, 1024])
+@pytest.mark.parametrize("threads", [64, 256])
+def test_every_launch_shape_meets_parity(chunk, threads, identity_mode):
+    """Every launch shape takes the same step: strictly within 1e-5 of the oracle in the physical
+    identity mode; in the reference identity mode, whose F the oracle itself reproduces only to its
+    reorder noise (the same step with the particles in another order), within max(1e-5, 2 x that
+    noise) per field."""
+    sc, mid, ref, floor = _launch_shape_case(identity_mode)
+    with MPMSimulation(mid, sc.material, res=64, p2g_chunk=chunk, g2p_threads=threads,
+                       identity_mode=identity_mode) as sim:
         assert sim.tuning() == {"p2g_chunk": chunk, "g2p_threads": threads, "compact": 0}
         sim.advance(1)
         out = sim.particles()
-    ref = O.run(to_oracle(mid), int(sc.material), res=64, steps=1)
-    errs = assert_parity(out, ref, inv_dx=64.0, label=f"chunk {chunk} threads {threads}")
-    print(f"[tuning] chunk={chunk} threads={threads} {errs}", flush=True)
+    label = f"chunk {chunk} threads {threads} idm{identity_mode}"
+    if identity_mode == 1:
+        errs = assert_parity(out, ref, inv_dx=64.0, label=label)
+    else:
+        errs = assert_parity_with_floor(out, ref, floor, inv_dx=64.0, label=label)
+    print(f"[tuning] {label} {errs} floor {floor}", flush=True)
 
 
 def test_autotune_keeps_state_and_picks_a_valid_shape():

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from parity import TOL, assert_parity, assert_parity_with_floor, grid_errors, reorder_noise
+from parity import LIGHT_NODE, TOL, assert_parity, assert_parity_with_floor, grid_errors, reorder_noise
 
 from nuclearmpm_amd import MaterialModel, MPMSimulation, Particles, scenes
 
@@ -92,9 +92,12 @@ def test_grid_matches_oracle(dim):
     errs = grid_errors(gv, gm, rv, rm)
     print(f"[parity] grid dim={dim} {errs} active={active}", flush=True)
     assert errs["grid_v"] <= TOL and errs["grid_m"] <= TOL, errs
-    assert ((gm > 0) == (rm.reshape(gm.shape) > 0)).all()
+    # the same nodes carry mass, except that a contribution below the fixed-point quantum (about
+    # 1e-15 of the work item's largest) rounds to zero: such a node is lighter than LIGHT_NODE
+    rmg = rm.reshape(gm.shape)
+    assert (rmg[(gm > 0) != (rmg > 0)] <= LIGHT_NODE * rmg.max()).all()
     # active_nodes counts post-P2G mass > 0, before the sticky box zeroes boundary nodes
-    assert active >= int((rm > 0).sum())
+    assert active >= int((rm > LIGHT_NODE * rm.max()).sum())
 
 
 def test_full_size_four_cubes_3d():
