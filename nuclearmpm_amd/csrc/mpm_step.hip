@@ -250,6 +250,11 @@ __global__ __launch_bounds__(256) void k_p2g_tile(const float* __restrict__ st, 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
+        // an empty asm that takes the fractional offsets as operands: the per-node weight products
+        // of the deposit cannot be computed before the barrier above.  Hoisted there, with one
+        // particle per thread, they held the kernel at 256 VGPRs (one wave per SIMD); now 66-110.
+#pragma unroll
+        for (int a = 0; a < D; ++a) asm volatile("" : "+v"(fxs[k][a]));
         if (finite[k])
         {
             if constexpr (D == 3 && MODE == kSvdAccurate)
@@ -329,15 +334,8 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
         first[c] = s_first[c];
         count[c] = s_count[c];
     }
-    // NPT nodes per thread, one after the other.  For each node the first work item of every
-    // candidate tile is loaded before any sum (a candidate whose region misses the node contributes
-    // +0.0, which leaves a sum that started at +0.0 bitwise unchanged), so those loads are in
-    // flight together; then the sums run in the fixed order candidate by candidate, items in order.
-    constexpr int NPT = TL::NODES / 256;
-    static_assert(TL::NODES % 256 == 0, "a node tile is a whole number of 256-thread passes");
     uint32_t massive_nodes = 0;
-    for (int k = 0; k < NPT; ++k) {
-        const int l = int(threadIdx.x) + k * 256;
+    for (int l = threadIdx.x; l < TL::NODES; l += blockDim.x) {
         int n[D];
         {
             int rem = l;
@@ -347,11 +345,11 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
                 rem >>= TL::LOG;
             }
         }
-        int roff[NC];  // slot offset of the node in candidate c's region, -1 outside it
-        float4 s1[NC];
+        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            bool inside = first[c] >= 0;
+            if (first[c] < 0) continue;
+            bool inside = true;
             int r = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
@@ -359,25 +357,35 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
                 inside = inside && ra < TL::R;
                 r = r * TL::R + ra;
             }
-            roff[c] = inside ? r : -1;
-            s1[c] = inside ? slots[int64_t(first[c]) * TL::REGION + r] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            g.x += s1[c].x;
-            g.y += s1[c].y;
-            g.z += s1[c].z;
-            g.w += s1[c].w;
-            if (roff[c] < 0 || count[c] < 2) continue;
-            // the tile's further work items, in order
-            const float4* q = slots + int64_t(first[c] + 1) * TL::REGION + roff[c];
-            for (int it = 1; it < count[c]; ++it, q += TL::REGION) {
-                const float4 sv = q[0];
-                g.x += sv.x;
-                g.y += sv.y;
-                g.z += sv.z;
-                g.w += sv.w;
+            if (!inside) continue;
+            // items in order; four loads in flight before their (ordered) sums
+            const float4* q = slots + int64_t(first[c]) * TL::REGION + r;
+            int it = 0;
+            for (; it + 4 <= count[c]; it += 4, q += 4 * TL::REGION) {
+                const float4 s0 = q[0], s1 = q[TL::REGION], s2 = q[2 * TL::REGION], s3 = q[3 * TL::REGION];
+                g.x += s0.x;
+                g.y += s0.y;
+                g.z += s0.z;
+                g.w += s0.w;
+                g.x += s1.x;
+                g.y += s1.y;
+                g.z += s1.z;
+                g.w += s1.w;
+                g.x += s2.x;
+                g.y += s2.y;
+                g.z += s2.z;
+                g.w += s2.w;
+                g.x += s3.x;
+                g.y += s3.y;
+                g.z += s3.z;
+                g.w += s3.w;
+            }
+            for (; it < count[c]; ++it, q += TL::REGION) {
+                const float4 s = q[0];
+                g.x += s.x;
+                g.y += s.y;
+                g.z += s.z;
+                g.w += s.w;
             }
         }
         massive_nodes += g.w > 0.0f ? 1u : 0u;
