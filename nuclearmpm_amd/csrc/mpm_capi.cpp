@@ -139,7 +139,8 @@ struct nclr_mpm_sim {
     int compact_wg_limit = 0;         // co-resident workgroups of the compact step
     float4 *cslots = nullptr, *cgrid = nullptr;
     uint8_t* cmassive = nullptr;
-    int *cbox = nullptr, *cinfo = nullptr, *csubbox = nullptr;
+    int *cbox = nullptr, *cinfo = nullptr;
+    unsigned long long* csubbox = nullptr;
     uint32_t* cbar = nullptr;
     int64_t steps = 0;
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
@@ -320,6 +321,9 @@ struct nclr_mpm_sim {
         if (!valid_compact(dim, ppt))
             return fail(MPM_ERR_INVALID, dim == 2 ? "compact must be 0, 1, 2 or 4" : "compact must be 0, 1 or 2 in 3D");
         if (ppt && scenes.size() != 1) return fail(MPM_ERR_UNSUPPORTED, "the compact step runs single-scene engines");
+        if (ppt && scenes[0].res > nclr_dev::kCompactMaxRes)
+            return fail(MPM_ERR_UNSUPPORTED,
+                        "the compact step runs grids up to res " + std::to_string(nclr_dev::kCompactMaxRes));
         if (ppt) {
             int limit = 0;
             RC_TRY(compact_limit_checked(limit));
@@ -341,7 +345,7 @@ struct nclr_mpm_sim {
         if (dmalloc(&cslots, 2 * size_t(limit) * cap) != hipSuccess || dmalloc(&cgrid, cap) != hipSuccess ||
             dmalloc(&cmassive, cap) != hipSuccess || dmalloc(&cbox, size_t(2 * dim)) != hipSuccess ||
             dmalloc(&cinfo, size_t(nclr_dev::kInfoWords)) != hipSuccess ||
-            dmalloc(&csubbox, size_t(2 * nclr_dev::kCompactMaxWorkgroups * 6)) != hipSuccess ||
+            dmalloc(&csubbox, size_t(nclr_dev::kCompactTableWords)) != hipSuccess ||
             dmalloc(&cbar, size_t(2)) != hipSuccess)
             return fail(MPM_ERR_HIP, "compact step: hipMalloc failed");
         return MPM_OK;
@@ -615,7 +619,9 @@ struct nclr_mpm_sim {
         // the compact step (single scenes small enough for one co-resident launch)
         if (rc == MPM_OK) rc = set_tuning(best_chunk, best_threads);
         for (int ppt : {1, 2, 4}) {
-            if (rc != MPM_OK || scenes.size() != 1 || !valid_compact(dim, ppt)) break;
+            if (rc != MPM_OK || scenes.size() != 1 || !valid_compact(dim, ppt) ||
+                scenes[0].res > nclr_dev::kCompactMaxRes)
+                break;
             int limit = 0;
             rc = compact_limit_checked(limit);
             if (rc != MPM_OK || compact_workgroups(ppt) > limit) continue;
@@ -908,7 +914,8 @@ struct nclr_mpm_sim {
         scene_flags = scene_frozen = nullptr;
         cslots = cgrid = nullptr;
         cmassive = nullptr;
-        cbox = cinfo = csubbox = nullptr;
+        cbox = cinfo = nullptr;
+        csubbox = nullptr;
         cbar = nullptr;
         p.tile_scene = p.node_tile_scene = nullptr;
         p.scene_flags = p.scene_frozen = nullptr;

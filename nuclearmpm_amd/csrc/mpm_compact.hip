@@ -8,8 +8,10 @@
 // workgroups whose sub-boxes hold the node, in workgroup order, applies grid_op
 // (src/nclr.h:263-310) into LDS and runs G2P (src/nclr.h:312-360) on its particles in place from
 // there.  A node's sum has the same terms in the same order in every workgroup that forms it, so
-// the step is deterministic.  One barrier per step, one launch for all steps; slots and sub-boxes
-// alternate by step parity, so a workgroup may deposit step s + 1 while another still reads step s.
+// the step is deterministic.  One barrier per step, one launch for all steps: each workgroup
+// publishes its sub-box and flags in a table entry stamped with the barrier's count, and the wait for
+// every entry is also their read.  Slots alternate by step parity and the tables rotate over three,
+// so a workgroup may deposit step s + 1 while another still reads step s.
 //
 // The box of step s: the stencil bases of the particles before step s lie within one cell of
 // those before step s - 1 (a particle moves less than one cell per step: the grid velocity is
@@ -113,8 +115,11 @@ NCLR_DI bool compact_barrier(const CompactArgs& a, uint32_t& k, uint32_t* s_flag
     return *s_flag != 0u;
 }
 
-__global__ void k_compact_init(int* __restrict__ box, uint32_t* __restrict__ bar, int* __restrict__ info, int dim) {
+__global__ void k_compact_init(int* __restrict__ box, uint32_t* __restrict__ bar, int* __restrict__ info,
+                               unsigned long long* __restrict__ subbox, int dim) {
     const int t = int(threadIdx.x);
+    // stamp 0 in every table word: no step of the launch matches it (its stamps count from 1)
+    for (int r = t; r < kCompactTableWords; r += blockDim.x) subbox[r] = 0ull;
     if (t < kInfoWords) info[t] = 0;
     if (t < 2) bar[t] = 0u;
     if (t < dim) {  // the base range before the first step
@@ -220,10 +225,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     extern __shared__ unsigned long long lds[];
     __shared__ float s_bound[4][4];
     __shared__ int s_red[4][6];                      // per wave: a range of bases or nodes
-    __shared__ int s_sb[6];                          // this workgroup's sub-box
+    __shared__ int s_sb[7];                          // this workgroup's sub-box and flags
     __shared__ int s_tab[kCompactMaxWorkgroups][6];  // sub-boxes of the overlapping (or all) workgroups
     __shared__ int s_list[kCompactMaxWorkgroups];    // their workgroup indices, ascending
     __shared__ int s_wcount[4];
+    __shared__ int s_wflags[4];                      // per wave: off the grid (1), outside the box (2)
     __shared__ int s_box[2 * D];
     __shared__ uint32_t s_flag;
     const SceneParams sp = a.sp;
@@ -234,6 +240,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     float* st = a.st;
     const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
     uint32_t k_bar = 0;
+    // the step barriers of this launch (the stamp of their table entries) and the table of the last
+    // completed step
+    uint32_t kb = 0;
+    int last_tab = 0;
+    // a scene stopped before the launch (read before the first barrier: no step has run yet)
+    const bool pre_stopped = (a.step.scene_flags[0] | a.step.scene_frozen[0]) != 0u;
     int64_t whole_nodes = 1;
 #pragma unroll
     for (int d = 0; d < D; ++d) whole_nodes *= sp.res + 1;
@@ -391,8 +403,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             finite[k] = finite[k] && bm < 3.0e38f;
             if (bm < 3.0e38f) bound[3] = fmaxf(bound[3], bm);
         }
-        if (off_grid) stop_scene(a.step, 0, kStatusOutOfRange);
+        if (off_grid) stop_scene(a.step, 0, kStatusOutOfRange);  // for the host; the table carries it too
         if (outside) atomicOr(&a.info[kInfoOutside], 1);
+        {
+            const unsigned long long mo = __ballot(off_grid), mu = __ballot(outside);
+            if (lane == 0) s_wflags[wave] = (mo != 0ull ? 1 : 0) | (mu != 0ull ? 2 : 0);
+        }
         // workgroup maxima of the bounds and the base range (this workgroup's sub-box)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -435,6 +451,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 s_sb[d] = l;
                 s_sb[3 + d] = h;
             }
+            s_sb[6] = s_wflags[0] | s_wflags[1] | s_wflags[2] | s_wflags[3];
         }
         float scale[4];
         double inv_scale[4];
@@ -485,47 +502,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                                                 : ((bad >> c) & 1u ? nan : fixed_point_value(acc[c * nodes + r], inv_scale[c]));
                 slots[r] = make_float4(out[0], out[1], out[2], out[3]);
             }
-            if (threadIdx.x < 6) a.subbox[(par * kCompactMaxWorkgroups + wg) * 6 + threadIdx.x] = s_sb[threadIdx.x];
         }
-        alive = compact_barrier(a, k_bar, &s_flag);
-        if (!alive) break;
 
-        // ---- stop or retry ------------------------------------------------------------------------
-        if ((a.step.scene_flags[0] | a.step.scene_frozen[0]) != 0u) {
-            // a particle off its grid: the scene stops with every particle as before this step;
-            // the stamp counts this and the remaining steps, as the tiled step's does
-            if (wg == 0 && threadIdx.x == 0) {
-                if (a.step.scene_frozen[0] == 0u) a.step.scene_frozen[0] = *a.stamp + 1u;
-                *a.stamp = *a.stamp + uint32_t(a.nsteps - step);
+        // ---- the step barrier: publish this workgroup's table entry, wait for every entry -------
+        // Each workgroup writes its sub-box and flags into the table of this barrier's parity, every
+        // 32-bit half of the entry stamped with the barrier's count (after a release fence, so its
+        // slots are visible first), and threads t < nwg poll entry t until all its halves carry the
+        // stamp.  The wait and the read of the entries are one round trip; a half with another
+        // stamp is an older barrier's (or the launch's zeroed table).  Three tables rotate with the
+        // barrier count: a workgroup one barrier ahead never overwrites an entry another still
+        // polls, and the last completed step's table survives the (at most two) barriers that can
+        // follow it, a step found outside its box and its retry, for the export after the loop.
+        // Bounded like compact_barrier: past ~2 s it raises kStatusBarrierTimeout.
+        ++kb;
+        const int tab = int(kb % 3u);
+        const uint32_t stamp16 = kb & 0xffffu;
+        __syncthreads();  // every slot store of the workgroup is issued
+        if (threadIdx.x == 0) {
+            __threadfence();
+            unsigned long long* e = a.subbox + (int64_t(tab) * kCompactMaxWorkgroups + wg) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t v0 = uint32_t(s_sb[2 * q]) & 0xffffu;
+                const uint32_t v1 = 2 * q + 1 < 7 ? uint32_t(s_sb[2 * q + 1]) & 0xffffu : 0u;
+                const unsigned long long word = (static_cast<unsigned long long>((stamp16 << 16) | v1) << 32) |
+                                                ((stamp16 << 16) | v0);
+                __hip_atomic_store(&e[q], word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            reason = kCompactStopped;
-            break;
         }
-        if (!whole && a.info[kInfoOutside] != 0) {
-            // a particle outside the box (the word stays raised: every workgroup decides alike)
-            if (!whole_fits) {
-                reason = kCompactOverflow;
-                break;
+        const int t = int(threadIdx.x);
+        int tb[7] = {1, 1, 1, 0, 0, 0, 0};  // empty
+        bool polled = true;
+        if (t < nwg) {
+            const unsigned long long* e = a.subbox + (int64_t(tab) * kCompactMaxWorkgroups + t) * 4;
+            uint32_t spins = 0;
+            for (;;) {
+                unsigned long long v[4];
+                bool all = true;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    v[q] = __hip_atomic_load(&e[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    all = all && (uint32_t(v[q]) >> 16) == stamp16 && (uint32_t(v[q] >> 32) >> 16) == stamp16;
+                }
+                if (all) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        tb[2 * q] = int(uint32_t(v[q]) & 0xffffu);
+                        if (2 * q + 1 < 7) tb[2 * q + 1] = int(uint32_t(v[q] >> 32) & 0xffffu);
+                    }
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kBarrierSpinLimit) {
+                    atomicOr(a.status, kStatusBarrierTimeout);
+                    polled = false;
+                    break;
+                }
             }
-            whole = true;
-            continue;
+            __threadfence();  // acquire: workgroup t's slots after its entry
         }
-        if (wg == 0 && threadIdx.x == 0) {
-            gstamp = *a.stamp + 1u;
-            *a.stamp = gstamp;
-        }
+        alive = __syncthreads_and(polled ? 1 : 0) != 0;
+        if (threadIdx.x == 0) __threadfence();
+        __syncthreads();
+        if (!alive) break;
 
         // ---- every workgroup's sub-box: the overlap list (ascending: the order of the sums) and
         //      their union, the stencil nodes before this step; the next step's box is the union
-        //      widened by one node (a particle moves less than one cell per step) ----------------
+        //      widened by one node (a particle moves less than one cell per step); and the flags of
+        //      all workgroups ---------------------------------------------------------------------
+        int flags = 0;
         {
-            const int t = int(threadIdx.x);
             bool hit = false;
-            int tb[6];
             int ul[3] = {INT_MAX, INT_MAX, INT_MAX}, uh[3] = {INT_MIN, INT_MIN, INT_MIN};
             if (t < nwg) {
-#pragma unroll
-                for (int e = 0; e < 6; ++e) tb[e] = a.subbox[(par * kCompactMaxWorkgroups + t) * 6 + e];
                 bool nonempty = true;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) nonempty = nonempty && tb[d] <= tb[3 + d];
@@ -548,8 +597,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 }
             }
             const unsigned long long m = __ballot(hit);
+            const unsigned long long mo = __ballot((tb[6] & 1) != 0), mu = __ballot((tb[6] & 2) != 0);
             if (lane == 0) {
                 s_wcount[wave] = __popcll(m);
+                s_wflags[wave] = (mo != 0ull ? 1 : 0) | (mu != 0ull ? 2 : 0);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     s_red[wave][d] = ul[d];
@@ -557,6 +608,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 }
             }
             __syncthreads();
+            flags = s_wflags[0] | s_wflags[1] | s_wflags[2] | s_wflags[3];
             int pos = __popcll(m & ((1ull << lane) - 1ull));
             for (int w = 0; w < wave; ++w) pos += s_wcount[w];
             if (hit) {
@@ -572,6 +624,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 nhi[d] = l <= h ? h + 1 : 0;
             }
             __syncthreads();
+        }
+
+        // ---- stop or retry (the same decision in every workgroup: the same entries) -------------
+        if (pre_stopped || (flags & 1)) {
+            // a particle off its grid: the scene stops with every particle as before this step;
+            // the stamp counts this and the remaining steps, as the tiled step's does
+            if (wg == 0 && threadIdx.x == 0) {
+                if (a.step.scene_frozen[0] == 0u) a.step.scene_frozen[0] = *a.stamp + 1u;
+                *a.stamp = *a.stamp + uint32_t(a.nsteps - step);
+            }
+            reason = kCompactStopped;
+            break;
+        }
+        if (!whole && (flags & 2)) {
+            // a particle outside the box
+            if (!whole_fits) {
+                reason = kCompactOverflow;
+                break;
+            }
+            whole = true;
+            continue;
+        }
+        if (wg == 0 && threadIdx.x == 0) {
+            gstamp = *a.stamp + 1u;
+            *a.stamp = gstamp;
         }
         const int nlist = s_wcount[0] + s_wcount[1] + s_wcount[2] + s_wcount[3];
 
@@ -612,6 +689,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             gext[d] = ext[d];
         }
         gnodes = nodes;
+        last_tab = tab;
 
         // ---- G2P from LDS ---------------------------------------------------------------------------
 #pragma unroll
@@ -647,9 +725,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     // ---- the grid of the last completed step over its whole box, for export ----------------------
     if (done > 0 && alive && compact_barrier(a, k_bar, &s_flag)) {
         const int par = (done - 1) & 1;
-        if (int(threadIdx.x) < nwg)
+        if (int(threadIdx.x) < nwg) {
+            const unsigned long long* e = a.subbox + (int64_t(last_tab) * kCompactMaxWorkgroups + threadIdx.x) * 4;
 #pragma unroll
-            for (int e = 0; e < 6; ++e) s_tab[threadIdx.x][e] = a.subbox[(par * kCompactMaxWorkgroups + threadIdx.x) * 6 + e];
+            for (int q = 0; q < 3; ++q) {
+                const unsigned long long v = e[q];
+                s_tab[threadIdx.x][2 * q] = int(uint32_t(v) & 0xffffu);
+                s_tab[threadIdx.x][2 * q + 1] = int(uint32_t(v >> 32) & 0xffffu);
+            }
+        }
         __syncthreads();
         const float4* slot0 = a.slots + int64_t(par) * a.slot_sets * CP::CAP;
         for (int r = wg * 256 + int(threadIdx.x); r < gnodes; r += nwg * 256) {
@@ -829,7 +913,7 @@ static hipError_t compact_one(const CompactArgs& a, int nwg, hipStream_t s) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        kCompactLdsBytes);
     if (e != hipSuccess) return e;
-    k_compact_init<<<1, 64, 0, s>>>(a.box, a.bar, a.info, D);
+    k_compact_init<<<1, 256, 0, s>>>(a.box, a.bar, a.info, a.subbox, D);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     CompactArgs args = a;
@@ -867,7 +951,8 @@ static hipError_t compact_dim(const CompactArgs& a, int material, int mode, int 
 
 hipError_t launch_compact_steps(int dim, int material, int mode, int ppt, int nwg, const CompactArgs& a, hipStream_t s) {
     const bool ppt_ok = ppt == 1 || ppt == 2 || (dim == 2 && ppt == 4);
-    if (!ppt_ok || nwg < 1 || nwg > kCompactMaxWorkgroups || int64_t(nwg) * 256 * ppt < a.n || a.n <= 0)
+    if (!ppt_ok || nwg < 1 || nwg > kCompactMaxWorkgroups || int64_t(nwg) * 256 * ppt < a.n || a.n <= 0 ||
+        a.sp.res > kCompactMaxRes)
         return hipErrorInvalidValue;
     return dim == 2 ? compact_dim<2>(a, material, mode, ppt, nwg, s) : compact_dim<3>(a, material, mode, ppt, nwg, s);
 }

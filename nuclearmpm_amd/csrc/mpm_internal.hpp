@@ -83,6 +83,8 @@ struct StepParams {
 // The compact step (mpm_compact.hip): dynamic LDS per workgroup for its box of nodes.
 constexpr int kCompactLdsBytes = 150 * 1024;
 constexpr int kCompactMaxWorkgroups = 256;
+constexpr int kCompactTableWords = 3 * kCompactMaxWorkgroups * 4;  // 64-bit words of the sub-box tables
+constexpr int kCompactMaxRes = 65000;  // node coordinates travel in 16 bits in the sub-box table
 
 // Words of CompactArgs::info, written by the compact step at the end of its launch.
 enum CompactInfo : int {
@@ -113,7 +115,10 @@ struct CompactArgs {
     uint32_t* stamp;       // device step stamp: + 1 per completed step
     uint32_t* status;      // engine status word (kStatusBarrierTimeout)
     float4* slots;         // 2 x slot_sets x box capacity: each workgroup's deposits, by step parity
-    int* subbox;           // 2 x kCompactMaxWorkgroups x [lo[3], hi[3]]: nodes each workgroup deposited to
+    // 3 x kCompactMaxWorkgroups x 4 words, by step barrier count mod 3: each workgroup's sub-box
+    // [lo[3], hi[3]] (the nodes it deposited to) and its flags, one 32-bit half per value, each half
+    // (stamp << 16) | value
+    unsigned long long* subbox;
     float4* grid;          // box capacity: the updated grid of the box of the last completed step
     uint8_t* massive;      // box capacity: 1 where that step's node mass after P2G is > 0
     int* box;              // [min[D], max[D]]: base range of the positions before the first step
