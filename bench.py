@@ -201,9 +201,11 @@ def batched_vs_sequential(steps: int):
     }
 
 
-def fwd_bwd(which: str):
+def fwd_bwd(which: str, svd: str = "exact"):
     """Forward + backward of a differentiable jelly rollout with checkpoints (torch.autograd over
-    the engine's adjoint kernels); particle-steps/s counts each simulated step once."""
+    the engine's adjoint kernels); particle-steps/s counts each simulated step once.  `svd`: the
+    engine's SVD mode (the accurate mode takes the polar factor of the forward and of the adjoint
+    from the FP64 Newton iteration)."""
     import numpy as np
     import torch
 
@@ -213,11 +215,12 @@ def fwd_bwd(which: str):
         specs = [scenes.jelly_2d_diff()]
         # the forward steps take the autotuner's step (the compact step, for this small scene); the
         # adjoint's recomputed steps are tiled (the adjoint kernels walk the tile sort)
-        sim = MPMSimulation(specs[0].particles, specs[0].material, res=specs[0].res, E=specs[0].E, autotune=True)
+        sim = MPMSimulation(specs[0].particles, specs[0].material, res=specs[0].res, E=specs[0].E, autotune=True,
+                            svd=svd)
         desc = "jelly_2d_diff: 140x140 particles at [0.3,0.5]^2, res 128, 512 steps, checkpoint every 64"
     else:
         specs = scenes.jelly_3d_batched(seed=2, scenes=8)
-        sim = MPMBatch(specs)
+        sim = MPMBatch(specs, svd=svd)
         desc = "jelly_3d_batched: 8 scenes x 64^3 particles, res 128, 256 steps, checkpoint every 32"
     steps, every = specs[0].steps, int(specs[0].extra["checkpoint_every"])
     d = specs[0].particles.dim
@@ -278,6 +281,7 @@ def fwd_bwd(which: str):
         "adjoint_kernels": kernels,
         "g_active_end": g_active,
         "tuning": tuning,
+        "svd": SVD_MODES[svd],
     }
 
 

@@ -673,7 +673,7 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
 // P2G adjoint (src/nclr.h:104-165, 313-372) per sorted position j.  Two waves per SIMD, as the
 // G2P adjoint (1.38x faster than one).
 // ---------------------------------------------------------------------------------------------
-template <int D, bool EXACT>
+template <int D, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p2g_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ pid,
                                                      const uint32_t* __restrict__ perm,
                                                      const uint32_t* __restrict__ keys, StepParams p,
@@ -715,7 +715,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     float dw[D][3];
 #pragma unroll
     for (int a = 0; a < D; ++a) bspline_dweights(fx[a], dw[a]);
-    const M<D> A = fused_affine<D, EXACT ? kSvdExact : kSvdFast>(sp.material, F, C, Jp, mass, vol, sp);
+    // the forward's affine matrix; in the accurate mode its F - R is kept, and the stress backward
+    // below takes R = F - (F - R) (within one float rounding of the iteration's R) instead of a
+    // second Newton iteration
+    M<D> A, FmR;
+    if constexpr (D == 3 && MODE == kSvdAccurate) {
+        FmR = f_minus_rotation<MODE, D>(F);
+        A = fused_affine_from<D>(sp.material, F, FmR, C, Jp, mass, vol, sp);
+    } else {
+        A = fused_affine<D, MODE>(sp.material, F, C, Jp, mass, vol, sp);
+    }
     M<D> Abar;
     float fxbar[D], vin[D];
 #pragma unroll
@@ -781,7 +790,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             sG += G.a[a][b];
         }
     // PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry
-    const M<D> R = polar_rotation_mode<EXACT ? kSvdExact : kSvdFast>(F);
+    M<D> R;
+    if constexpr (D == 3 && MODE == kSvdAccurate) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) R.a[a][b] = F.a[a][b] - FmR.a[a][b];
+    } else {
+        R = rotation_mode<MODE>(F);
+    }
     const float J = det(F);
     M<D> Fbar, Rbar;
     float mubar = 0.0f;
@@ -966,13 +983,16 @@ hipError_t launch_grid_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
 hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
     if (a.p.n > 0) {
         if (a.dim == 2)
-            k_p2g_adjoint<2, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
+            k_p2g_adjoint<2, kSvdExact><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
                 a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
-        else if ((a.svd_mode != 1))
-            k_p2g_adjoint<3, true><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
+        else if (a.svd_mode == kSvdAccurate)
+            k_p2g_adjoint<3, kSvdAccurate><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
+                a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
+        else if (a.svd_mode != kSvdFast)
+            k_p2g_adjoint<3, kSvdExact><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
                 a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
         else
-            k_p2g_adjoint<3, false><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
+            k_p2g_adjoint<3, kSvdFast><<<blocks_of(a.p.n, 256), 256, 0, a.stream>>>(
                 a.state_in, a.pid_in, a.perm, a.keys, a.p, b.gbar, b.grad_out, b.grad_in, b.gstride, b.epart);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -86,14 +86,13 @@ NCLR_DI float hardening(int material, float Jp) {
 // first_piola_kirchoff_stress (src/nclr.h:313-337): -(dt V0)(4 inv_dx^2 PF) + m C with
 // PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry.  MODE: the SVD mode of the polar factor
 // (kSvdExact: the reference's float arithmetic bit for bit).
-template <int D, int MODE>
-NCLR_DI M<D> fused_affine(int material, const M<D>& F, const M<D>& C, float Jp, float mass, float vol,
-                          const SceneParams& sp) {
+template <int D>
+NCLR_DI M<D> fused_affine_from(int material, const M<D>& F, const M<D>& diff, const M<D>& C, float Jp, float mass,
+                               float vol, const SceneParams& sp) {
     const float e = hardening(material, Jp);
     const float mu = sp.mu0 * e;
     const float lambda = sp.lambda0 * e;
     const float J = det(F);
-    const M<D> diff = f_minus_rotation<MODE, D>(F);
     const float Dinv = 4.0f * sp.inv_dx * sp.inv_dx;
     const float two_mu = 2.0f * mu;
     M<D> FmR;
@@ -110,6 +109,11 @@ NCLR_DI M<D> fused_affine(int material, const M<D>& F, const M<D>& C, float Jp, 
 #pragma unroll
         for (int b = 0; b < D; ++b) A.a[a][b] = neg_dtv * (Dinv * (PF.a[a][b] + lam_term)) + mass * C.a[a][b];
     return A;
+}
+template <int D, int MODE>
+NCLR_DI M<D> fused_affine(int material, const M<D>& F, const M<D>& C, float Jp, float mass, float vol,
+                          const SceneParams& sp) {
+    return fused_affine_from<D>(material, F, f_minus_rotation<MODE, D>(F), C, Jp, mass, vol, sp);
 }
 
 template <int D>

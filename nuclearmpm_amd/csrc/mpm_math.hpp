@@ -427,13 +427,13 @@ NCLR_DI void cofactors(const double (&X)[3][3], double (&c)[3][3]) {
     c[2][2] = fma(X[0][0], X[1][1], -X[0][1] * X[1][0]);
 }
 
-// F - R, R the polar rotation of F (the reference's nclr_polar, src/nclr_math.h:76-98, for det F
-// > 0), by Newton's iteration X <- (X + X^-T) / 2 = (X + cof(X) / det X) / 2 from X = F in double,
-// for F with ||F||_F^3 <= 40 det F (condition number below ~8; false otherwise, and the caller
-// takes the Jacobi SVD).  The iteration converges quadratically to R; once a step changes X by at
-// most 1e-6 one more step leaves an error below 1e-12, and F - R is rounded once to float, so it
-// keeps its relative precision at small strain.
-NCLR_DI bool f_minus_polar_newton(const M<3>& F, M<3>& out) {
+// F - R and R, R the polar rotation of F (the reference's nclr_polar,
+// src/nclr_math.h:76-98, for det F > 0), by Newton's iteration X <- (X + X^-T) / 2 =
+// (X + cof(X) / det X) / 2 from X = F in double, for F with ||F||_F^3 <= 40 det F (condition number
+// below ~8; false otherwise, and the caller takes the Jacobi SVD).  The iteration converges
+// quadratically to R; once a step changes X by at most 1e-6 one more step leaves an error below
+// 1e-12, and F - R is rounded once to float, so it keeps its relative precision at small strain.
+NCLR_DI bool polar_newton(const M<3>& F, M<3>& fmr, M<3>& rot) {
     double X[3][3], c[3][3];
     double nf2 = 0.0;
 #pragma unroll
@@ -470,8 +470,17 @@ NCLR_DI bool f_minus_polar_newton(const M<3>& F, M<3>& out) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) out.a[i][j] = float(double(F.a[i][j]) - fma(c[i][j], half_inv, 0.5 * X[i][j]));
+        for (int j = 0; j < 3; ++j) {
+            const double r = fma(c[i][j], half_inv, 0.5 * X[i][j]);
+            fmr.a[i][j] = float(double(F.a[i][j]) - r);
+            rot.a[i][j] = float(r);
+        }
     return true;
+}
+template <bool ROTATION = false>
+NCLR_DI bool f_minus_polar_newton(const M<3>& F, M<3>& out) {
+    M<3> unused;
+    return ROTATION ? polar_newton(F, unused, out) : polar_newton(F, out, unused);
 }
 
 // One rotation of the cyclic Jacobi eigen-decomposition of the symmetric B (Golub & Van Loan,
@@ -675,6 +684,17 @@ NCLR_DI M<D> f_minus_rotation(const M<D>& F) {
 #pragma unroll
         for (int b = 0; b < D; ++b) out.a[a][b] = F.a[a][b] - R.a[a][b];
     return out;
+}
+
+// R with nclr_polar's convention: in the accurate mode (3D) Newton's iteration where it applies
+// and the two-sided Jacobi elsewhere, as f_minus_rotation; otherwise the mode's SVD.
+template <int MODE, int D>
+NCLR_DI M<D> rotation_mode(const M<D>& F) {
+    if constexpr (D == 3 && MODE == kSvdAccurate) {
+        M<D> R;
+        if (f_minus_polar_newton<true>(F, R)) return R;
+    }
+    return polar_rotation_mode<MODE == kSvdAccurate ? kSvdExact : MODE>(F);
 }
 
 // Quadratic B-spline weights (src/nclr.h:124-127).

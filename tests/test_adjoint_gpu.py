@@ -48,14 +48,16 @@ def rel_err(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-@pytest.mark.parametrize("d,side,res", [(2, 24, 64), (3, 10, 32)])
-def test_one_step_vjp_matches_fp64_oracle_adjoint(d, side, res):
+@pytest.mark.parametrize("d,side,res,svd", [(2, 24, 64, "exact"), (3, 10, 32, "exact"), (3, 10, 32, "accurate")])
+def test_one_step_vjp_matches_fp64_oracle_adjoint(d, side, res, svd):
+    """The accurate SVD mode's adjoint takes its polar factor from the FP64 Newton iteration of its
+    forward step; both modes are held to the FP64 oracle adjoint."""
     p = jelly_particles(d, side, seed=d)
     n = p.n
     rng = np.random.default_rng(11)
     cot = {k: rng.normal(size=getattr(p, k).shape) for k in FIELDS}
-    print(f"[adjoint] one-step VJP d={d} n={n} res={res}", flush=True)
-    with MPMSimulation(p, MaterialModel.kJelly, res=res, E=1e4) as sim:
+    print(f"[adjoint] one-step VJP d={d} n={n} res={res} svd={svd}", flush=True)
+    with MPMSimulation(p, MaterialModel.kJelly, res=res, E=1e4, svd=svd) as sim:
         s0 = to_state(p).requires_grad_(True)
         E = torch.tensor([1e4], dtype=torch.float32)
         E.requires_grad_(True)
@@ -350,8 +352,8 @@ MATS = {"snow": (MaterialModel.kSnow, O.SNOW), "liquid": (MaterialModel.kLiquid,
 
 
 @pytest.mark.parametrize("mat", ["snow", "liquid"])
-@pytest.mark.parametrize("d,side,res", [(2, 20, 64), (3, 8, 32)])
-def test_one_step_vjp_snow_liquid_matches_fp64_oracle_adjoint(mat, d, side, res):
+@pytest.mark.parametrize("d,side,res,svd", [(2, 20, 64, "exact"), (3, 8, 32, "exact"), (3, 8, 32, "accurate")])
+def test_one_step_vjp_snow_liquid_matches_fp64_oracle_adjoint(mat, d, side, res, svd):
     """The G2P adjoint through snow's SVD clamp and Jp update and liquid's determinant reset, and
     the P2G adjoint through snow's hardening, against the FP64 oracle adjoint on the same state."""
     em, om = MATS[mat]
@@ -359,8 +361,8 @@ def test_one_step_vjp_snow_liquid_matches_fp64_oracle_adjoint(mat, d, side, res)
     n = p.n
     rng = np.random.default_rng(17)
     cot = {k: rng.normal(size=getattr(p, k).shape) for k in FIELDS + ("Jp",)}
-    print(f"[adjoint] one-step VJP {mat} d={d} n={n} res={res}", flush=True)
-    with MPMSimulation(p, em, res=res, E=1e4, identity_mode=1) as sim:
+    print(f"[adjoint] one-step VJP {mat} d={d} n={n} res={res} svd={svd}", flush=True)
+    with MPMSimulation(p, em, res=res, E=1e4, identity_mode=1, svd=svd) as sim:
         s0 = autodiff.pack(*(torch.from_numpy(getattr(p, k)).cuda() for k in FIELDS),
                            Jp=torch.from_numpy(p.Jp).cuda()).requires_grad_(True)
         E = torch.tensor([1e4], dtype=torch.float64, requires_grad=True)
