@@ -70,7 +70,8 @@ typedef struct mpm_options {
      * 2 = accurate: F - R by Newton's polar iteration in FP64 (two-sided Jacobi where det F <= 0 or
      * F is ill-conditioned) and the snow / liquid SVD by an FP64 Jacobi eigen-decomposition of
      * A^T A; closer to the FP64 step than the reference's FP32 arithmetic is, and faster than 0.
-     * The adjoint kernels use mode 0's factors for modes 0 and 2. */
+     * The P2G adjoint takes mode 2's Newton polar factor in mode 2; the G2P adjoint takes mode 0's
+     * factors for modes 0 and 2. */
     int32_t fast_svd;
     /* Single-scene engines: the compact step with this many particles per thread (1, 2 or 4; 3D: 1
      * or 2), 0 = off (the default).  See mpm_set_compact. */
