@@ -580,36 +580,38 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                 r = r * TL::R + ra;
             }
             if (!inside) continue;
-            // items in order (the forward grid kernel's order); two items' loads in flight
-            auto add = [&](const float4 s, const float4 sb) {
-                g.x += s.x;
-                g.y += s.y;
-                g.z += s.z;
-                g.w += s.w;
-                b.x += sb.x;
-                b.y += sb.y;
-                b.z += sb.z;
-            };
-            auto add_b = [&](const float4 sb) {
-                b.x += sb.x;
-                b.y += sb.y;
-                b.z += sb.z;
-            };
+            // items in order (the forward grid kernel's order)
+            // rounds of loads in flight before their (ordered) sums; an item past the count adds
+            // +0.0, which leaves a sum that started at +0.0 bitwise unchanged
             const int64_t q = int64_t(first[c]) * TL::REGION + r;
-            int it = 0;
-            if (gpre) {  // the forward sums are kept: only the adjoint slots
-                for (; it + 4 <= count[c]; it += 4) {
-                    const int64_t s0 = q + int64_t(it) * TL::REGION;
-                    const float4 b0 = slots_b[s0], b1 = slots_b[s0 + TL::REGION], b2 = slots_b[s0 + 2 * TL::REGION],
-                                 b3 = slots_b[s0 + 3 * TL::REGION];
-                    add_b(b0);
-                    add_b(b1);
-                    add_b(b2);
-                    add_b(b3);
+            if (gpre) {  // the forward sums are kept: only the adjoint slots, eight per round
+                const float4* qb = slots_b + q;
+                for (int it = 0; it < count[c]; it += 8, qb += 8 * TL::REGION) {
+                    float4 bv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        bv[u] = it + u < count[c] ? qb[u * TL::REGION] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        b.x += bv[u].x;
+                        b.y += bv[u].y;
+                        b.z += bv[u].z;
+                        b.w += bv[u].w;
+                    }
                 }
-                for (; it < count[c]; ++it) add_b(slots_b[q + int64_t(it) * TL::REGION]);
                 continue;
             }
+            // without the tape: both slot sets, two items' four loads in flight
+            auto add = [&](const float4 sa, const float4 sb) {
+                g.x += sa.x;
+                g.y += sa.y;
+                g.z += sa.z;
+                g.w += sa.w;
+                b.x += sb.x;
+                b.y += sb.y;
+                b.z += sb.z;
+            };
+            int it = 0;
             for (; it + 2 <= count[c]; it += 2) {
                 const int64_t s0 = q + int64_t(it) * TL::REGION, s1 = s0 + TL::REGION;
                 const float4 a0 = slots[s0], b0 = slots_b[s0], a1 = slots[s1], b1 = slots_b[s1];
