@@ -248,11 +248,12 @@ NCLR_DI float ldg(const float* __restrict__ g, int64_t gstride, int c, int64_t j
 
 // ---------------------------------------------------------------------------------------------
 // G2P adjoint (src/nclr.h:167-258).  GENERAL: any material (the scene's, at run time); otherwise
-// every scene is jelly.  At least two waves per SIMD (at most 256 VGPRs): measured 1.45x faster
-// than one wave per SIMD.
+// every scene is jelly.  A work item of TPB * PPT particles, PPT per thread.  At least two waves per
+// SIMD (at most 256 VGPRs): measured 1.45x faster than one wave per SIMD; four (128 VGPRs) with one
+// 3D particle per thread.
 // ---------------------------------------------------------------------------------------------
-template <int D, int PPT, bool GENERAL, bool EXACT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
+template <int D, int PPT, int TPB, bool GENERAL, bool EXACT>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(D == 3 && PPT == 1 ? 4 : 2))) void k_g2p_adjoint(const float* __restrict__ st, const uint32_t* __restrict__ perm,
                                                      const uint32_t* __restrict__ item_offset, StepParams p,
                                                      const float4* __restrict__ gv, const float* __restrict__ gout,
                                                      float* __restrict__ gin, int64_t gstride,
@@ -263,7 +264,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int K = D == 2 ? 9 : 27;
     __shared__ unsigned long long acc[D * REG];
     __shared__ uint32_t nonfinite[REG];
-    __shared__ float s_bound[4][D];
+    constexpr int NW = TPB / 64;
+    __shared__ float s_bound[NW][D];
     const uint32_t item = blockIdx.x;
     const WorkItem wi = work_item(p, item);  // loaded with the count below, used only if valid
     if (item >= item_offset[p.ntiles]) return;
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int c = 0; c < D; ++c) bound[c] = 0.0f;
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
+        const uint32_t j = j0 + uint32_t(k) * TPB + threadIdx.x;
         valid[k] = false;
         if (j >= j1) continue;
         const int64_t i = perm[j];
@@ -405,16 +407,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     float scale[D];
     double inv_scale[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c)
-        fixed_point_scale(fmaxf(fmaxf(s_bound[0][c], s_bound[1][c]), fmaxf(s_bound[2][c], s_bound[3][c])), scale[c],
-                          inv_scale[c]);
+    for (int c = 0; c < D; ++c) {
+        float b = s_bound[0][c];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) b = fmaxf(b, s_bound[w][c]);
+        fixed_point_scale(b, scale[c], inv_scale[c]);
+    }
 
     // pass 2: gather the updated grid (for the forward C and the weight derivatives), scatter the
     // quantised grid cotangent, write the partial x and F cotangents
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
-        const uint32_t j = j0 + uint32_t(k) * 256 + threadIdx.x;
+        const uint32_t j = j0 + uint32_t(k) * TPB + threadIdx.x;
         float w[D][3], dw[D][3];
         int rb = 0;
 #pragma unroll
@@ -942,28 +947,40 @@ using namespace nclr_dev;
 
 static inline unsigned blocks_of(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
 
-template <int D, int PPT, bool GENERAL, bool EXACT>
+template <int D, int PPT, int TPB, bool GENERAL, bool EXACT>
 static hipError_t g2p_adjoint_one(const LaunchArgs& a, const AdjointArgs& b) {
-    const int64_t items = (a.p.n + 256 * PPT - 1) / (256 * PPT) + a.p.ntiles;
-    k_g2p_adjoint<D, PPT, GENERAL, EXACT><<<unsigned(items), 256, 0, a.stream>>>(
+    // the forward's work items: TPB * PPT particles each
+    const int64_t items = (a.p.n + TPB * PPT - 1) / (TPB * PPT) + a.p.ntiles;
+    k_g2p_adjoint<D, PPT, TPB, GENERAL, EXACT><<<unsigned(items), TPB, 0, a.stream>>>(
         a.state_in, a.perm, a.item_offset, a.p, a.gv, b.grad_out, b.grad_in, b.gstride, b.slots_b);
     return hipGetLastError();
 }
 
-template <int D, int PPT>
+template <int D, int PPT, int TPB>
 static hipError_t g2p_adjoint_material(const LaunchArgs& a, const AdjointArgs& b) {
-    if (a.material == MPM_JELLY) return g2p_adjoint_one<D, PPT, false, true>(a, b);
+    if (a.material == MPM_JELLY) return g2p_adjoint_one<D, PPT, TPB, false, true>(a, b);
     // 2D SVDs have one method; in 3D the forward's choice
-    if (D == 2 || (a.svd_mode != 1)) return g2p_adjoint_one<D, PPT, true, true>(a, b);
-    return g2p_adjoint_one<D, PPT, true, false>(a, b);
+    if (D == 2 || (a.svd_mode != 1)) return g2p_adjoint_one<D, PPT, TPB, true, true>(a, b);
+    return g2p_adjoint_one<D, PPT, TPB, true, false>(a, b);
 }
 
+// 2D: 256 threads, chunk / 256 particles each.  3D: one particle per thread (a workgroup of chunk
+// threads), at most 128 VGPRs and four waves per SIMD, where two particles per thread need 144
+// and fit three.
 template <int D>
 static hipError_t g2p_adjoint_dispatch(const LaunchArgs& a, const AdjointArgs& b) {
-    switch (a.p.chunk) {
-        case 256: return g2p_adjoint_material<D, 1>(a, b);
-        case 1024: return g2p_adjoint_material<D, 4>(a, b);
-        default: return g2p_adjoint_material<D, 2>(a, b);
+    if constexpr (D == 3) {
+        switch (a.p.chunk) {
+            case 256: return g2p_adjoint_material<D, 1, 256>(a, b);
+            case 1024: return g2p_adjoint_material<D, 1, 1024>(a, b);
+            default: return g2p_adjoint_material<D, 1, 512>(a, b);
+        }
+    } else {
+        switch (a.p.chunk) {
+            case 256: return g2p_adjoint_material<D, 1, 256>(a, b);
+            case 1024: return g2p_adjoint_material<D, 4, 256>(a, b);
+            default: return g2p_adjoint_material<D, 2, 256>(a, b);
+        }
     }
 }
 

@@ -120,9 +120,11 @@ struct nclr_mpm_sim {
     float* gtmp[2] = {nullptr, nullptr};
     char* tape = nullptr;  // saved states of a backward segment
     // The sort's outputs (keys, perm, tile bounds, item offsets, the item table, meta) live in one
-    // block, so a backward segment can keep each recomputed step's sort with one copy.
+    // block at fixed offsets, so a backward segment can run each recomputed step's sort in a block
+    // of its own (a sort tape entry) and the adjoint kernels read it there.
     char* sort_block = nullptr;
     size_t sort_block_bytes = 0;
+    size_t so_keys = 0, so_perm = 0, so_ts = 0, so_te = 0, so_io = 0, so_items = 0, so_meta = 0;
     char* sort_tape = nullptr;  // per recomputed step of a backward segment, its sort block
     int64_t tape_states = 0;
     // per step of a backward segment, the forward grid (gv) and its pre-update sums (gpre): the
@@ -698,14 +700,26 @@ struct nclr_mpm_sim {
         const hipError_t e = hipMalloc(reinterpret_cast<void**>(&sort_block), off);
         if (e != hipSuccess) return e;
         sort_block_bytes = off;
-        keys = reinterpret_cast<uint32_t*>(sort_block + o_keys);
-        perm = reinterpret_cast<uint32_t*>(sort_block + o_perm);
-        tile_start = reinterpret_cast<uint32_t*>(sort_block + o_ts);
-        tile_end = reinterpret_cast<uint32_t*>(sort_block + o_te);
-        item_offset = reinterpret_cast<uint32_t*>(sort_block + o_io);
-        p.items = reinterpret_cast<uint4*>(sort_block + o_items);
-        meta = reinterpret_cast<uint32_t*>(sort_block + o_meta);
+        so_keys = o_keys;
+        so_perm = o_perm;
+        so_ts = o_ts;
+        so_te = o_te;
+        so_io = o_io;
+        so_items = o_items;
+        so_meta = o_meta;
+        point_sort_block(sort_block);
         return hipSuccess;
+    }
+
+    // The sort's pointers into the block at `base` (the engine's own or a sort tape entry).
+    void point_sort_block(char* base) {
+        keys = reinterpret_cast<uint32_t*>(base + so_keys);
+        perm = reinterpret_cast<uint32_t*>(base + so_perm);
+        tile_start = reinterpret_cast<uint32_t*>(base + so_ts);
+        tile_end = reinterpret_cast<uint32_t*>(base + so_te);
+        item_offset = reinterpret_cast<uint32_t*>(base + so_io);
+        p.items = reinterpret_cast<uint4*>(base + so_items);
+        meta = reinterpret_cast<uint32_t*>(base + so_meta);
     }
 
     int ensure_adjoint() {
@@ -765,12 +779,14 @@ struct nclr_mpm_sim {
     // segment are recomputed straight into the tape: state t (t >= 1) lives in tape entry t - 1,
     // laid out as save_state writes it, and state 0 stays in the engine's own buffer.  The steps
     // and the adjoint steps run on those buffers in place (the state pointers are redirected, no
-    // copies).  Each recomputed step's sort block is kept in the sort tape and restored for that
-    // step's adjoint instead of sorting again, and, with the grid tape, each step's forward grid and
-    // pre-update sums are written straight into their tape entries, so the adjoint steps run only
-    // the adjoint kernels.  The engine's own buffers, pointers and step counter are restored at the
-    // end, so the state is the checkpoint again.  Graph replay is off inside
-    // the segment (graphs hold the engine's own pointers).
+    // copies).  Each recomputed step sorts into its own sort tape entry (the sort pointers are
+    // redirected; the incremental sort's inputs from the step before, the item offsets, the item
+    // table and meta, are copied in first) and that step's adjoint reads the entry in place
+    // instead of sorting again.  With the grid tape, each step's forward grid and pre-update sums
+    // are written straight into their tape entries, so the adjoint steps run only the adjoint
+    // kernels.  The engine's own buffers, pointers and step counter are restored at the end, so
+    // the state is the checkpoint again.  Graph replay is off inside the segment (graphs hold the
+    // engine's own pointers).
     int backward_segment(int nsteps, const float* gout, float* gin, double* grad_E) {
         RC_TRY(ensure_adjoint());
         if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
@@ -830,6 +846,7 @@ struct nclr_mpm_sim {
                 e.sorted_valid = false;
                 e.gv = gv;
                 e.gpre = nullptr;
+                e.point_sort_block(e.sort_block);
             }
         } restore{*this, {state[0], state[1]}, {pid[0], pid[1]}, cur, steps, opts.graph, gv};
         // the grid tape entry of step t; own: the engine's own grid (with the tape's pre-update sums)
@@ -848,14 +865,19 @@ struct nclr_mpm_sim {
                 pid[slot] = reinterpret_cast<uint32_t*>(entry + state_floats() * sizeof(float));
             }
         };
-        // step t sorts state t: its sort block is kept for the adjoint of step t
+        // step t sorts state t into sort tape entry t, kept for the adjoint of step t
+        auto sort_entry = [&](int t) { return sort_tape + sort_block_bytes * size_t(t); };
         for (int t = 0; t + 1 < nsteps; ++t) {
             point_at(cur ^ 1, t + 1);
             grid_at(t, false);
+            char* entry = sort_entry(t);
+            if (sorted_valid)  // the incremental sort reads the item offsets, items and meta it rewrites
+                HIP_TRY(hipMemcpyAsync(entry + so_io, (t == 0 ? sort_block : sort_entry(t - 1)) + so_io,
+                                       sort_block_bytes - so_io, hipMemcpyDeviceToDevice, stream));
+            point_sort_block(entry);
             RC_TRY(step());
-            HIP_TRY(hipMemcpyAsync(sort_tape + sort_block_bytes * size_t(t), sort_block, sort_block_bytes,
-                                   hipMemcpyDeviceToDevice, stream));
         }
+        point_sort_block(sort_block);
         const size_t gbytes = size_t(grad_channels()) * size_t(p.n) * sizeof(float);
         HIP_TRY(hipMemcpyAsync(gtmp[0], gout, gbytes, hipMemcpyDeviceToDevice, stream));
         int c = 0;
@@ -863,9 +885,7 @@ struct nclr_mpm_sim {
             point_at(cur, t);
             sorted_valid = false;
             const bool kept = t + 1 < nsteps;  // the last state was never stepped, so never sorted
-            if (kept)
-                HIP_TRY(hipMemcpyAsync(sort_block, sort_tape + sort_block_bytes * size_t(t), sort_block_bytes,
-                                       hipMemcpyDeviceToDevice, stream));
+            point_sort_block(kept ? sort_entry(t) : sort_block);
             // the segment's last adjoint step (t = 0) recomputes its grid into the engine's own grid,
             // so the grid export and active-node count afterwards show a step from the checkpoint
             grid_at(t, t == 0);
