@@ -210,6 +210,7 @@ def fwd_bwd(which: str, svd: str = "exact"):
     import torch
 
     from nuclearmpm_amd import MPMBatch, MPMSimulation, autodiff, scenes
+    from nuclearmpm_amd._lib import lib
 
     if which == "jelly_2d_diff":
         specs = [scenes.jelly_2d_diff()]
@@ -231,10 +232,10 @@ def fwd_bwd(which: str, svd: str = "exact"):
                     for s in specs], dim=1)
     E = torch.tensor([s.E for s in specs], dtype=torch.float64)
 
-    def run():
+    def run(record="none"):
         s0 = x0.clone().requires_grad_(True)
         e = E.clone().requires_grad_(True)
-        out = autodiff.rollout(sim, s0, e, steps, every)
+        out = autodiff.rollout(sim, s0, e, steps, every, record=record)
         xs = autodiff.unpack(out, d)[0]
         loss = 0.0
         start = 0
@@ -254,6 +255,28 @@ def fwd_bwd(which: str, svd: str = "exact"):
         autodiff.rollout(sim, x0, E, steps, every)
     torch.cuda.synchronize()
     tf = time.perf_counter() - t0
+    # the same rollout with its segments recorded in the forward pass (record="auto": as many as fit
+    # in half the free device memory), so that the backward pass does not recompute them
+    run("auto")  # warm-up (the tapes' allocations)
+    plan = autodiff._record_plan(lib(), sim._handle(), steps, every, "auto", x0.device)
+    rec_bytes = sum(int(lib().mpm_record_bytes(sim._handle(), min(every, steps - i * every), m))
+                    for i, m in enumerate(plan) if m is not None)
+    t0 = time.perf_counter()
+    _, g_rec, ge_rec = run("auto")
+    t_rec = time.perf_counter() - t0
+    recorded = {
+        "value": n * steps / t_rec,
+        "segments": len(plan),
+        "recorded_with_grids": sum(m == 1 for m in plan),
+        "recorded_without_grids": sum(m == 0 for m in plan),
+        "tape_GB": rec_bytes / 1e9,
+        "gradients_bitwise_equal": bool(torch.equal(g_rec, g)) and bool(torch.equal(ge_rec, ge)),
+        # a single-scene engine's plain forward may take the compact step, whose fixed-point sums
+        # round differently from the tiled step's that recorded segments (and recomputations) take
+        "grad_max_rel_diff": float((g_rec - g).abs().max() / g.abs().max().clamp_min(1e-30)),
+        "gradE_max_rel_diff": float((ge_rec - ge).abs().max() / ge.abs().max().clamp_min(1e-30)),
+    }
+    del g_rec, ge_rec
     # a separate pass with HIP events per kernel: the adjoint kernels' rooflines
     sim.set_timing(True)
     sim.reset_kernel_times()
@@ -276,6 +299,7 @@ def fwd_bwd(which: str, svd: str = "exact"):
         "value": n * steps / t,
         "unit": "particle-steps/s (forward + backward)",
         "forward_only": n * steps / tf,
+        "recorded": recorded,
         "loss": loss,
         "grad_finite": bool(torch.isfinite(g).all()) and bool(torch.isfinite(ge).all()),
         "adjoint_kernels": kernels,

@@ -268,6 +268,29 @@ int mpm_backward_segment(mpm_handle h, int nsteps, const float* grad_out, float*
  * recomputes its grid). */
 int64_t mpm_tape_bytes(mpm_handle h, int nsteps);
 
+/* Recorded segments: no counterpart in the reference (whose checkpointing recomputes every
+ * segment); for callers with device memory to spare, which trade it for the recomputation.
+ *
+ * mpm_record_bytes: device bytes of the tape of one recorded segment of `nsteps` steps (-1 for a
+ * NULL handle or nsteps < 1): nsteps - 1 state snapshots and sorts as in mpm_tape_bytes, and with
+ * with_grid != 0 nsteps forward grids and pre-update sums.
+ *
+ * mpm_advance_recorded: advances `nsteps` steps from the current state like mpm_advance (on the
+ * tiled step) and records into `tape` (caller-owned device memory of mpm_record_bytes bytes,
+ * 256-byte aligned) what mpm_backward_segment would recompute from this state.
+ *
+ * mpm_backward_recorded: mpm_backward_segment for a segment recorded by mpm_advance_recorded, run
+ * from the segment's first state (restored by the caller, e.g. with mpm_load_state) without
+ * recomputing the recorded steps; the gradients are bitwise those of mpm_backward_segment.  The
+ * tape must be unchanged since its recording, and MPM_ERR_INVALID is returned for a tape that was
+ * not recorded on this engine with these nsteps and with_grid, or when the tuning or a scene's
+ * elasticity changed since.  The tape may be passed again (a second backward pass) until it is
+ * recorded over or freed. */
+int64_t mpm_record_bytes(mpm_handle h, int nsteps, int with_grid);
+int mpm_advance_recorded(mpm_handle h, int nsteps, void* tape, int with_grid);
+int mpm_backward_recorded(mpm_handle h, int nsteps, void* tape, int with_grid, const float* grad_out,
+                          float* grad_in, double* grad_E);
+
 /* Sets Young's modulus and Poisson's ratio of one scene (mu_0, lambda_0 recomputed in float as
  * src/nclr.h:77-78 does). */
 int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu);

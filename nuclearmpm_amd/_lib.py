@@ -159,6 +159,12 @@ def lib() -> ctypes.CDLL:
     L.mpm_backward_segment.restype = i32
     L.mpm_tape_bytes.argtypes = [H, i32]
     L.mpm_tape_bytes.restype = i64
+    L.mpm_record_bytes.argtypes = [H, i32, i32]
+    L.mpm_record_bytes.restype = i64
+    L.mpm_advance_recorded.argtypes = [H, i32, _vp, i32]
+    L.mpm_advance_recorded.restype = i32
+    L.mpm_backward_recorded.argtypes = [H, i32, _vp, i32, _vp, _vp, _vp]
+    L.mpm_backward_recorded.restype = i32
     L.mpm_set_scene_elasticity.argtypes = [H, i32, f32, f32]
     L.mpm_set_scene_elasticity.restype = i32
     u32x4 = ctypes.POINTER(ctypes.c_uint32)

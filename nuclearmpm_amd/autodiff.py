@@ -8,6 +8,11 @@ clamp's singular-value backward, grid update, P2G with the polar-decomposition b
 snow hardening) step by step in reverse.  Gradients flow to the initial x, v, F, C, Jp of every
 particle and to Young's modulus of every scene.
 
+With `record`, segments are recorded in the forward pass instead (mpm_advance_recorded: the
+states, sorts and optionally grids the recomputation would produce, on a device tape per segment)
+and their backward skips the recomputation (mpm_backward_recorded); the gradients are bitwise the
+same, the device memory grows by one tape per recorded segment.
+
 State tensors are (channels, n) float32 on the engine's device: rows x[d], v[d], F[d*d], C[d*d]
 (matrices row-major), Jp, particles in the order the simulation was created with.
 """
@@ -53,9 +58,36 @@ def _ready(t: torch.Tensor) -> None:
     torch.cuda.current_stream(t.device).synchronize()
 
 
+RECORD_MODES = ("none", "auto", "grid", "nogrid")
+
+
+def _record_plan(L, h, steps: int, every: int, record: str, device) -> list:
+    """Per segment: None (checkpoint only, recomputed in the backward pass), 0 (recorded without
+    grids) or 1 (recorded with grids).  "auto" records as many segments as fit in half the device
+    memory that is free or cached by torch, first without grids, then adds grids while they fit."""
+    sizes = [min(every, steps - d) for d in range(0, steps, every)]
+    if record == "none":
+        return [None] * len(sizes)
+    if record in ("grid", "nogrid"):
+        return [1 if record == "grid" else 0] * len(sizes)
+    free, _ = torch.cuda.mem_get_info(device)
+    budget = 0.5 * (free + torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device))
+    plan, used = [None] * len(sizes), 0
+    for i, k in enumerate(sizes):
+        b = int(L.mpm_record_bytes(h, k, 0))
+        if used + b <= budget:
+            plan[i], used = 0, used + b
+    for i, k in enumerate(sizes):
+        if plan[i] == 0:
+            extra = int(L.mpm_record_bytes(h, k, 1)) - int(L.mpm_record_bytes(h, k, 0))
+            if used + extra <= budget:
+                plan[i], used = 1, used + extra
+    return plan
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, state0: torch.Tensor, E: torch.Tensor, sim: _Engine, steps: int, every: int):
+    def forward(ctx, state0: torch.Tensor, E: torch.Tensor, sim: _Engine, steps: int, every: int, record: str):
         L, h = lib(), sim._handle()
         if state0.shape != (channels(sim.dim), sim.n) or state0.dtype != torch.float32 or not state0.is_cuda:
             raise ValueError("state0 must be a (channels, n) float32 tensor on the engine's device")
@@ -70,14 +102,20 @@ class _Rollout(torch.autograd.Function):
             sim.set_elasticity(s, e)
         check(L.mpm_set_state_device(h, state0.data_ptr()))
         sb = int(L.mpm_state_bytes(h))
+        plan = _record_plan(L, h, steps, every, record, state0.device)
         ckpts = []
         done = 0
-        while done < steps:
+        for mode in plan:
             k = min(every, steps - done)
             buf = torch.empty(sb, dtype=torch.uint8, device=state0.device)
             check(L.mpm_save_state(h, buf.data_ptr()))
-            check(L.mpm_advance(h, k))
-            ckpts.append((buf, k))
+            rec = None
+            if mode is None:
+                check(L.mpm_advance(h, k))
+            else:
+                rec = torch.empty(int(L.mpm_record_bytes(h, k, mode)), dtype=torch.uint8, device=state0.device)
+                check(L.mpm_advance_recorded(h, k, rec.data_ptr(), mode))
+            ckpts.append((buf, k, rec, mode))
             done += k
         final = torch.empty(sb, dtype=torch.uint8, device=state0.device)
         check(L.mpm_save_state(h, final.data_ptr()))
@@ -107,20 +145,34 @@ class _Rollout(torch.autograd.Function):
         g = torch.empty_like(g_user)
         check(L.mpm_load_state(h, ctx.final.data_ptr()))
         check(L.mpm_grad_from_particle_order(h, g_user.data_ptr(), g.data_ptr()))
-        for buf, k in reversed(ctx.ckpts):
+        for buf, k, rec, mode in reversed(ctx.ckpts):
             check(L.mpm_load_state(h, buf.data_ptr()))
-            check(L.mpm_backward_segment(h, int(k), g.data_ptr(), g.data_ptr(), gE.data_ptr()))
+            if rec is None:
+                check(L.mpm_backward_segment(h, int(k), g.data_ptr(), g.data_ptr(), gE.data_ptr()))
+            else:
+                check(L.mpm_backward_recorded(h, int(k), rec.data_ptr(), mode, g.data_ptr(), g.data_ptr(),
+                                              gE.data_ptr()))
         g0 = torch.empty_like(g)
         check(L.mpm_grad_to_particle_order(h, g.data_ptr(), g0.data_ptr()))
         check(L.mpm_synchronize(h))
         sim._cache = None
         dtype, device, shape = ctx.E_meta
-        return g0, gE.to(dtype=dtype, device=device).reshape(shape), None, None, None
+        return g0, gE.to(dtype=dtype, device=device).reshape(shape), None, None, None, None
 
 
-def rollout(sim: _Engine, state0: torch.Tensor, E: torch.Tensor, steps: int, checkpoint_every: int = 32):
+def rollout(sim: _Engine, state0: torch.Tensor, E: torch.Tensor, steps: int, checkpoint_every: int = 32,
+            record: str = "none"):
     """Differentiable `steps`-step rollout of a simulation or batch (any materials) from `state0`
     ((channels, n) float32 on the engine's device) with Young's modulus E (one per scene).
     Returns the final state in the same layout.  The simulation is left at the rollout's final
-    state after the forward pass and at its initial state after the backward pass."""
-    return _Rollout.apply(state0, E, sim, int(steps), int(checkpoint_every))
+    state after the forward pass and at its initial state after the backward pass.
+
+    record: "none" saves a checkpoint every `checkpoint_every` steps and recomputes each segment in
+    the backward pass; "grid" / "nogrid" record every segment in the forward pass (with / without
+    its grids) so that the backward pass does not recompute it; "auto" records as many segments as
+    fit in half the free device memory.  Nothing is recorded when no gradient is needed."""
+    if record not in RECORD_MODES:
+        raise ValueError(f"record must be one of {RECORD_MODES}")
+    if not (torch.is_grad_enabled() and (state0.requires_grad or E.requires_grad)):
+        record = "none"
+    return _Rollout.apply(state0, E, sim, int(steps), int(checkpoint_every), record)

@@ -775,18 +775,120 @@ struct nclr_mpm_sim {
         return MPM_OK;
     }
 
-    // Adjoint of `nsteps` steps from the current state (a checkpoint).  The states inside the
-    // segment are recomputed straight into the tape: state t (t >= 1) lives in tape entry t - 1,
-    // laid out as save_state writes it, and state 0 stays in the engine's own buffer.  The steps
-    // and the adjoint steps run on those buffers in place (the state pointers are redirected, no
-    // copies).  Each recomputed step sorts into its own sort tape entry (the sort pointers are
-    // redirected; the incremental sort's inputs from the step before, the item offsets, the item
-    // table and meta, are copied in first) and that step's adjoint reads the entry in place
-    // instead of sorting again.  With the grid tape, each step's forward grid and pre-update sums
-    // are written straight into their tape entries, so the adjoint steps run only the adjoint
-    // kernels.  The engine's own buffers, pointers and step counter are restored at the end, so
-    // the state is the checkpoint again.  Graph replay is off inside the segment (graphs hold the
-    // engine's own pointers).
+    // Where a segment's steps live on a tape: state t (t >= 1) at states + (t - 1) * state_bytes(),
+    // laid out as save_state writes it (state 0 stays in the engine's own buffer); step t's sort
+    // block (t < nsteps - 1) at sorts + t * sort_block_bytes; with a grid tape (gv non-null), step
+    // t's forward grid and pre-update sums at gv + t * gv_nodes and gpre + t * gv_nodes (nsteps
+    // entries each: the last step's adjoint recomputes its grid into its entry).
+    struct SegTape {
+        char* states;
+        char* sorts;
+        float4* gv;
+        float4* gpre;
+    };
+
+    size_t gv_nodes() const { return size_t(p.node_tiles) * nclr_host::tile_nodes(dim); }
+
+    // The engine's own buffers and pointers for the length of a segment (graph replay off: graphs
+    // hold the engine's own pointers); the destructor puts them back.  Unless `advanced` is set,
+    // the step counter and the current state go back to the saved ones (the segment's first state);
+    // with it, the segment's last step stands: its state is in the own buffer state[cur ^ 1] saved.
+    struct SegGuard {
+        nclr_mpm_sim& e;
+        float* state[2];
+        uint32_t* pid[2];
+        int cur;
+        int64_t steps;
+        int graph;
+        float4* gv;
+        bool advanced = false;
+        explicit SegGuard(nclr_mpm_sim& s)
+            : e(s), state{s.state[0], s.state[1]}, pid{s.pid[0], s.pid[1]}, cur(s.cur), steps(s.steps),
+              graph(s.opts.graph), gv(s.gv) {
+            s.opts.graph = 0;
+        }
+        ~SegGuard() {
+            for (int k = 0; k < 2; ++k) {
+                e.state[k] = state[k];
+                e.pid[k] = pid[k];
+            }
+            if (advanced) {
+                e.cur = cur ^ 1;
+            } else {
+                e.cur = cur;
+                e.steps = steps;
+            }
+            e.opts.graph = graph;
+            e.sorted_valid = advanced;  // the own sort block holds the last step's sort
+            e.gv = gv;
+            e.gpre = nullptr;
+            e.point_sort_block(e.sort_block);
+        }
+    };
+
+    // slot <- state t of the segment on tape T
+    void point_state(int slot, int t, const SegTape& T, const SegGuard& g) {
+        if (t == 0) {
+            state[slot] = g.state[g.cur];
+            pid[slot] = g.pid[g.cur];
+        } else {
+            char* entry = T.states + state_bytes() * size_t(t - 1);
+            state[slot] = reinterpret_cast<float*>(entry);
+            pid[slot] = reinterpret_cast<uint32_t*>(entry + state_floats() * sizeof(float));
+        }
+    }
+
+    // Steps 0 .. nsteps - 2 of a segment from the current state, each into its tape entries: step t
+    // writes state t + 1, sorts state t into its own sort block (the sort pointers are redirected;
+    // the incremental sort's inputs from the step before, the item offsets, the item table and
+    // meta, are copied in first) and, with a grid tape, writes its grid and pre-update sums there.
+    // The state pointers are left on state nsteps - 1.
+    int record_steps(int nsteps, const SegTape& T, const SegGuard& g) {
+        const size_t gn = gv_nodes();
+        for (int t = 0; t + 1 < nsteps; ++t) {
+            point_state(cur ^ 1, t + 1, T, g);
+            gv = T.gv ? T.gv + size_t(t) * gn : g.gv;
+            gpre = T.gv ? T.gpre + size_t(t) * gn : nullptr;
+            char* entry = T.sorts + sort_block_bytes * size_t(t);
+            if (sorted_valid)
+                HIP_TRY(hipMemcpyAsync(entry + so_io,
+                                       (t == 0 ? sort_block : T.sorts + sort_block_bytes * size_t(t - 1)) + so_io,
+                                       sort_block_bytes - so_io, hipMemcpyDeviceToDevice, stream));
+            point_sort_block(entry);
+            RC_TRY(step());
+        }
+        return MPM_OK;
+    }
+
+    // The adjoint steps of a segment recorded on tape T, from state nsteps - 1 back to state 0, on
+    // the tape entries in place.  Steps with a recorded sort skip the sort, and with a recorded grid
+    // they run only the adjoint kernels.  The last state was never stepped, so its adjoint step
+    // sorts and recomputes its grid; the first one (t = 0) recomputes its grid into the engine's own
+    // grid, so the grid export and active-node count afterwards show a step from the segment's start.
+    int adjoint_steps(int nsteps, const SegTape& T, const SegGuard& g, const float* gout, float* gin,
+                      double* grad_E) {
+        const size_t gn = gv_nodes();
+        const size_t gbytes = size_t(grad_channels()) * size_t(p.n) * sizeof(float);
+        HIP_TRY(hipMemcpyAsync(gtmp[0], gout, gbytes, hipMemcpyDeviceToDevice, stream));
+        int c = 0;
+        for (int t = nsteps - 1; t >= 0; --t) {
+            point_state(cur, t, T, g);
+            sorted_valid = false;
+            const bool kept = t + 1 < nsteps;
+            point_sort_block(kept ? T.sorts + sort_block_bytes * size_t(t) : sort_block);
+            gv = T.gv && t != 0 ? T.gv + size_t(t) * gn : g.gv;
+            gpre = T.gv ? T.gpre + size_t(t) * gn : nullptr;
+            RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E, kept, kept && T.gv && t > 0));
+            c ^= 1;
+        }
+        HIP_TRY(hipMemcpyAsync(gin, gtmp[c], gbytes, hipMemcpyDeviceToDevice, stream));
+        return MPM_OK;
+    }
+
+    // Adjoint of `nsteps` steps from the current state (a checkpoint): the segment's states are
+    // recomputed straight into the engine's tape (record_steps) and the adjoint steps run on it
+    // (adjoint_steps); no state copies.  The engine's own buffers, pointers and step counter are
+    // restored at the end, so the state is the checkpoint again.
     int backward_segment(int nsteps, const float* gout, float* gin, double* grad_E) {
         RC_TRY(ensure_adjoint());
         if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
@@ -802,10 +904,9 @@ struct nclr_mpm_sim {
             HIP_TRY(hipMalloc(reinterpret_cast<void**>(&sort_tape), sort_block_bytes * size_t(nsteps - 1)));
             tape_states = nsteps - 1;
         }
-        const size_t gv_nodes = size_t(p.node_tiles) * nclr_host::tile_nodes(dim);
+        const size_t gn = gv_nodes();
         if (tape_grid_states < nsteps) {
-            // the grid tape: every step of the segment (the last one's adjoint recomputes its grid
-            // into its entry), if it leaves a quarter of the free device memory
+            // the grid tape: every step of the segment, if it leaves a quarter of the free device memory
             HIP_TRY(hipStreamSynchronize(stream));
             if (tape_gv) (void)hipFree(tape_gv);
             if (tape_gpre) (void)hipFree(tape_gpre);
@@ -813,9 +914,9 @@ struct nclr_mpm_sim {
             tape_grid_states = 0;
             size_t free_b = 0, total_b = 0;
             HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-            const size_t need = 2 * size_t(nsteps) * gv_nodes * sizeof(float4);
-            if (need <= free_b / 4 * 3 && dmalloc(&tape_gv, size_t(nsteps) * gv_nodes) == hipSuccess) {
-                if (dmalloc(&tape_gpre, size_t(nsteps) * gv_nodes) == hipSuccess) {
+            const size_t need = 2 * size_t(nsteps) * gn * sizeof(float4);
+            if (need <= free_b / 4 * 3 && dmalloc(&tape_gv, size_t(nsteps) * gn) == hipSuccess) {
+                if (dmalloc(&tape_gpre, size_t(nsteps) * gn) == hipSuccess) {
                     tape_grid_states = nsteps;
                 } else {
                     (void)hipFree(tape_gv);
@@ -827,73 +928,91 @@ struct nclr_mpm_sim {
             }
         }
         const bool grid_tape = tape_grid_states >= nsteps;
-        struct Restore {
-            nclr_mpm_sim& e;
-            float* state[2];
-            uint32_t* pid[2];
-            int cur;
-            int64_t steps;
-            int graph;
-            float4* gv;
-            ~Restore() {
-                for (int k = 0; k < 2; ++k) {
-                    e.state[k] = state[k];
-                    e.pid[k] = pid[k];
-                }
-                e.cur = cur;
-                e.steps = steps;
-                e.opts.graph = graph;
-                e.sorted_valid = false;
-                e.gv = gv;
-                e.gpre = nullptr;
-                e.point_sort_block(e.sort_block);
-            }
-        } restore{*this, {state[0], state[1]}, {pid[0], pid[1]}, cur, steps, opts.graph, gv};
-        // the grid tape entry of step t; own: the engine's own grid (with the tape's pre-update sums)
-        auto grid_at = [&](int t, bool own) {
-            gv = grid_tape && !own ? tape_gv + size_t(t) * gv_nodes : restore.gv;
-            gpre = grid_tape ? tape_gpre + size_t(t) * gv_nodes : nullptr;
+        const SegTape T{tape, sort_tape, grid_tape ? tape_gv : nullptr, grid_tape ? tape_gpre : nullptr};
+        SegGuard g(*this);
+        RC_TRY(record_steps(nsteps, T, g));
+        return adjoint_steps(nsteps, T, g, gout, gin, grad_E);
+    }
+
+    // A recorded segment's tape inside caller memory: states, sort blocks, then (with_grid) the
+    // forward grids and the pre-update sums, each part 256-byte aligned.
+    size_t record_layout(int nsteps, bool with_grid, char* base, SegTape* T) const {
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t o = off;
+            off += (bytes + 255) / 256 * 256;
+            return o;
         };
-        opts.graph = 0;
-        auto point_at = [&](int slot, int t) {  // slot <- state t of the segment
-            if (t == 0) {
-                state[slot] = restore.state[restore.cur];
-                pid[slot] = restore.pid[restore.cur];
-            } else {
-                char* entry = tape + sb * size_t(t - 1);
-                state[slot] = reinterpret_cast<float*>(entry);
-                pid[slot] = reinterpret_cast<uint32_t*>(entry + state_floats() * sizeof(float));
-            }
-        };
-        // step t sorts state t into sort tape entry t, kept for the adjoint of step t
-        auto sort_entry = [&](int t) { return sort_tape + sort_block_bytes * size_t(t); };
-        for (int t = 0; t + 1 < nsteps; ++t) {
-            point_at(cur ^ 1, t + 1);
-            grid_at(t, false);
-            char* entry = sort_entry(t);
-            if (sorted_valid)  // the incremental sort reads the item offsets, items and meta it rewrites
-                HIP_TRY(hipMemcpyAsync(entry + so_io, (t == 0 ? sort_block : sort_entry(t - 1)) + so_io,
+        const size_t o_states = take(state_bytes() * size_t(nsteps - 1));
+        const size_t o_sorts = take(sort_block_bytes * size_t(nsteps - 1));
+        const size_t grid_bytes = with_grid ? size_t(nsteps) * gv_nodes() * sizeof(float4) : 0;
+        const size_t o_gv = take(grid_bytes), o_gpre = take(grid_bytes);
+        if (T)
+            *T = SegTape{base + o_states, base + o_sorts,
+                         with_grid ? reinterpret_cast<float4*>(base + o_gv) : nullptr,
+                         with_grid ? reinterpret_cast<float4*>(base + o_gpre) : nullptr};
+        return off;
+    }
+
+    struct RecordInfo {
+        const void* base;
+        int nsteps;
+        bool with_grid;
+        int chunk;
+        std::vector<nclr_dev::SceneParams> scenes;
+    };
+    std::vector<RecordInfo> records;  // segments recorded by advance_recorded, most recent last
+    static constexpr size_t kMaxRecords = 4096;
+
+    // Advances `nsteps` steps from the current state and records on the caller's tape `rec`
+    // (record_layout bytes) what backward_segment would recompute from this state: the steps run
+    // as record_steps does, and the last one into the engine's own buffers, so the engine is left
+    // at the segment's last state like advance().  The sorts run on the tiled step's path.
+    int advance_recorded(int nsteps, char* rec, bool with_grid) {
+        if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
+        if (!rec) return fail(MPM_ERR_INVALID, "NULL tape");
+        SegTape T{};
+        (void)record_layout(nsteps, with_grid, rec, &T);
+        if (p.n == 0) {
+            RC_TRY(advance(nsteps));
+        } else {
+            SegGuard g(*this);
+            RC_TRY(record_steps(nsteps, T, g));
+            const int own = g.cur ^ 1;
+            state[cur ^ 1] = g.state[own];
+            pid[cur ^ 1] = g.pid[own];
+            gv = g.gv;
+            gpre = nullptr;
+            if (sorted_valid && nsteps > 1)
+                HIP_TRY(hipMemcpyAsync(sort_block + so_io, T.sorts + sort_block_bytes * size_t(nsteps - 2) + so_io,
                                        sort_block_bytes - so_io, hipMemcpyDeviceToDevice, stream));
-            point_sort_block(entry);
+            point_sort_block(sort_block);
             RC_TRY(step());
+            g.advanced = true;
         }
-        point_sort_block(sort_block);
-        const size_t gbytes = size_t(grad_channels()) * size_t(p.n) * sizeof(float);
-        HIP_TRY(hipMemcpyAsync(gtmp[0], gout, gbytes, hipMemcpyDeviceToDevice, stream));
-        int c = 0;
-        for (int t = nsteps - 1; t >= 0; --t) {
-            point_at(cur, t);
-            sorted_valid = false;
-            const bool kept = t + 1 < nsteps;  // the last state was never stepped, so never sorted
-            point_sort_block(kept ? sort_entry(t) : sort_block);
-            // the segment's last adjoint step (t = 0) recomputes its grid into the engine's own grid,
-            // so the grid export and active-node count afterwards show a step from the checkpoint
-            grid_at(t, t == 0);
-            RC_TRY(backward_step(gtmp[c], gtmp[c ^ 1], grad_E, kept, kept && grid_tape && t > 0));
-            c ^= 1;
-        }
-        HIP_TRY(hipMemcpyAsync(gin, gtmp[c], gbytes, hipMemcpyDeviceToDevice, stream));
+        records.erase(std::remove_if(records.begin(), records.end(), [&](const RecordInfo& r) { return r.base == rec; }),
+                      records.end());
+        if (records.size() >= kMaxRecords) records.erase(records.begin());
+        records.push_back(RecordInfo{rec, nsteps, with_grid, p.chunk, scenes});
         return MPM_OK;
+    }
+
+    // backward_segment over a segment recorded by advance_recorded from the current state (the
+    // caller restores the segment's first state): no recomputation of the recorded steps.
+    int backward_recorded(int nsteps, char* rec, bool with_grid, const float* gout, float* gin, double* grad_E) {
+        RC_TRY(ensure_adjoint());
+        if (nsteps < 1) return fail(MPM_ERR_INVALID, "nsteps must be >= 1");
+        auto it = std::find_if(records.begin(), records.end(), [&](const RecordInfo& r) { return r.base == rec; });
+        if (it == records.end() || it->nsteps != nsteps || it->with_grid != with_grid)
+            return fail(MPM_ERR_INVALID, "the tape was not recorded by mpm_advance_recorded with these nsteps and with_grid");
+        if (it->chunk != p.chunk || it->scenes.size() != scenes.size() ||
+            std::memcmp(it->scenes.data(), scenes.data(), scenes.size() * sizeof(scenes[0])) != 0)
+            return fail(MPM_ERR_INVALID, "the engine's parameters (tuning or elasticity) changed since the tape was recorded");
+        if (p.n == 0) return MPM_OK;
+        SegTape T{};
+        (void)record_layout(nsteps, with_grid, rec, &T);
+        SegGuard g(*this);
+        return adjoint_steps(nsteps, T, g, gout, gin, grad_E);
     }
 
     int set_elasticity(int s, float E, float nu) {
@@ -1490,6 +1609,25 @@ int64_t mpm_tape_bytes(mpm_handle h, int nsteps) {
     if (!h || nsteps < 1) return -1;
     const int64_t grid = 2 * int64_t(h->p.node_tiles) * int64_t(nclr_host::tile_nodes(h->dim)) * int64_t(sizeof(float4));
     return int64_t(nsteps - 1) * int64_t(h->state_bytes() + h->sort_block_bytes) + int64_t(nsteps) * grid;
+}
+
+int64_t mpm_record_bytes(mpm_handle h, int nsteps, int with_grid) {
+    if (!h || nsteps < 1) return -1;
+    return int64_t(h->record_layout(nsteps, with_grid != 0, nullptr, nullptr));
+}
+
+int mpm_advance_recorded(mpm_handle h, int nsteps, void* tape, int with_grid) {
+    if (!h) return fail(MPM_ERR_INVALID, "NULL handle");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    return h->advance_recorded(nsteps, static_cast<char*>(tape), with_grid != 0);
+}
+
+int mpm_backward_recorded(mpm_handle h, int nsteps, void* tape, int with_grid, const float* grad_out,
+                          float* grad_in, double* grad_E) {
+    if (!h || !tape || ((!grad_out || !grad_in) && h->p.n > 0)) return fail(MPM_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->opts.device));
+    RC_TRY(h->require_live());
+    return h->backward_recorded(nsteps, static_cast<char*>(tape), with_grid != 0, grad_out, grad_in, grad_E);
 }
 
 int mpm_set_scene_elasticity(mpm_handle h, int scene, float E, float nu) {

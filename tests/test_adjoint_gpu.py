@@ -298,6 +298,58 @@ def test_checkpoint_interval_does_not_change_gradients():
             np.testing.assert_array_equal(x, fresh.particles().x)
 
 
+@pytest.mark.parametrize("d", [2, 3])
+def test_recorded_segments_give_the_same_gradients(d):
+    """Segments recorded in the forward pass (with grids, without, or as many as fit) give the
+    recomputing checkpoint schedule's final state and gradients bit for bit, leave the engine where
+    it leaves it, and a recorded tape is refused for other nsteps or after the elasticity changed."""
+    if d == 2:
+        specs = [SceneSpec(Particles.from_positions(scenes.cube(2, 30, 0.3, 0.5)), MaterialModel.kJelly, 128, E=5e3)]
+        steps, every = 40, 7
+    else:
+        specs = [SceneSpec(s.particles, MaterialModel.kJelly, 48, E=s.E)
+                 for s in scenes.jelly_3d_batched(seed=4, scenes=2, side_res=10)]
+        steps, every = 20, 8
+    nps = [s.particles.n for s in specs]
+    targets = torch.from_numpy(np.random.default_rng(2).uniform(0.2, 0.8, (len(specs), d))).cuda()
+    results = []
+    for record, graph in (("none", False), ("grid", False), ("nogrid", False), ("auto", True), ("grid", True)):
+        print(f"[adjoint] recorded segments d={d} record={record} graph={graph}", flush=True)
+        with MPMBatch(specs, graph=graph) as b:
+            s0 = torch.cat([to_state(s.particles) for s in specs], dim=1).requires_grad_(True)
+            E = torch.tensor([s.E for s in specs], dtype=torch.float64, requires_grad=True)
+            out = autodiff.rollout(b, s0, E, steps, every, record=record)
+            fin = out.detach().cpu().numpy()
+            com_loss(out, d, targets, nps).backward()
+            x_after = np.concatenate([b.particles(s).x for s in range(len(specs))])
+            b.advance(3)
+            x3 = np.concatenate([b.particles(s).x for s in range(len(specs))])
+            results.append((fin, s0.grad.cpu().numpy(), E.grad.numpy(), x_after, x3))
+    for r in results[1:]:
+        for a, ref in zip(r, results[0]):
+            np.testing.assert_array_equal(a, ref)
+    np.testing.assert_array_equal(results[0][3], np.concatenate([s.particles.x for s in specs]))
+    # a tape is checked against its recording
+    L = lib()
+    with MPMBatch(specs) as b:
+        h = b._handle()
+        k = 4
+        tape = torch.empty(int(L.mpm_record_bytes(h, k, 1)), dtype=torch.uint8, device="cuda")
+        snap = torch.empty(int(L.mpm_state_bytes(h)), dtype=torch.uint8, device="cuda")
+        check(L.mpm_save_state(h, snap.data_ptr()))
+        check(L.mpm_advance_recorded(h, k, tape.data_ptr(), 1))
+        check(L.mpm_load_state(h, snap.data_ptr()))
+        gch = autodiff.channels(d)
+        g = torch.zeros(gch, b.n, device="cuda")
+        gE = torch.zeros(len(specs), dtype=torch.float64, device="cuda")
+        assert L.mpm_backward_recorded(h, k - 1, tape.data_ptr(), 1, g.data_ptr(), g.data_ptr(), gE.data_ptr()) != 0
+        assert L.mpm_backward_recorded(h, k, tape.data_ptr(), 0, g.data_ptr(), g.data_ptr(), gE.data_ptr()) != 0
+        b.set_elasticity(0, specs[0].E * 2)
+        assert L.mpm_backward_recorded(h, k, tape.data_ptr(), 1, g.data_ptr(), g.data_ptr(), gE.data_ptr()) != 0
+        b.set_elasticity(0, specs[0].E)
+        check(L.mpm_backward_recorded(h, k, tape.data_ptr(), 1, g.data_ptr(), g.data_ptr(), gE.data_ptr()))
+
+
 def test_batched_gradients_equal_single_scene_gradients():
     specs = [SceneSpec(s.particles, MaterialModel.kJelly, 48, E=s.E)
              for s in scenes.jelly_3d_batched(seed=4, scenes=3, side_res=10)]
