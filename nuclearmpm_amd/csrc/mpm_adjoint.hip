@@ -440,6 +440,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(D == 3 && P
         // 3D: unrolled by rows of three nodes (fully unrolled, the early node loads spill)
 #pragma unroll(D == 3 ? 3 : K)
         for (int o = 0; o < K; ++o) {
+            // products and sums contract to fused multiply-adds here (the kernel is bound by its
+            // VALU issue; these derivatives have no bitwise counterpart to match)
+#pragma clang fp contract(fast)
             int off[D];
             offsets<D>(o, off);
             float d[D];
@@ -722,16 +725,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     float dw[D][3];
 #pragma unroll
     for (int a = 0; a < D; ++a) bspline_dweights(fx[a], dw[a]);
-    // the forward's affine matrix; in the accurate mode its F - R is kept, and the stress backward
-    // below takes R = F - (F - R) (within one float rounding of the iteration's R) instead of a
-    // second Newton iteration
-    M<D> A, FmR;
+    // the forward's affine matrix from one polar factor: in the accurate mode its F - R is kept,
+    // and the stress backward below takes R = F - (F - R) (within one float rounding of the
+    // iteration's R) instead of a second Newton iteration; in the other modes R itself is kept and
+    // F - R formed as the forward forms it
+    M<D> A, FmR, R;
     if constexpr (D == 3 && MODE == kSvdAccurate) {
         FmR = f_minus_rotation<MODE, D>(F);
-        A = fused_affine_from<D>(sp.material, F, FmR, C, Jp, mass, vol, sp);
     } else {
-        A = fused_affine<D, MODE>(sp.material, F, C, Jp, mass, vol, sp);
+        R = rotation_mode<MODE, D>(F);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) FmR.a[a][b] = F.a[a][b] - R.a[a][b];
     }
+    A = fused_affine_from<D>(sp.material, F, FmR, C, Jp, mass, vol, sp);
     M<D> Abar;
     float fxbar[D], vin[D];
 #pragma unroll
@@ -745,6 +753,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     // 3D: unrolled by rows of three nodes (fully unrolled, the early node loads spill)
 #pragma unroll(D == 3 ? 3 : K)
     for (int o = 0; o < K; ++o) {
+        // fused multiply-adds here, as in the G2P adjoint's stencil loop
+#pragma clang fp contract(fast)
         int off[D];
         offsets<D>(o, off);
         float dpos[D];
@@ -797,14 +807,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             sG += G.a[a][b];
         }
     // PF = 2 mu (F - R) F^T + lambda (J - 1) J on every entry
-    M<D> R;
     if constexpr (D == 3 && MODE == kSvdAccurate) {
 #pragma unroll
         for (int a = 0; a < D; ++a)
 #pragma unroll
             for (int b = 0; b < D; ++b) R.a[a][b] = F.a[a][b] - FmR.a[a][b];
-    } else {
-        R = rotation_mode<MODE>(F);
     }
     const float J = det(F);
     M<D> Fbar, Rbar;
