@@ -932,21 +932,6 @@ __global__ __launch_bounds__(256) void k_scene_e_final(const double* __restrict_
     if (threadIdx.x == 0) grad_E[sc] += part[0];
 }
 
-// A single scene of at most kESingleMax particles: its range [0, meta[0]) summed by one workgroup
-// (strided per thread, then a fixed tree) straight into grad_E[0].
-constexpr int64_t kESingleMax = 32 * kEChunk;
-__global__ __launch_bounds__(256) void k_scene_e_single(const float* __restrict__ epart,
-                                                        const uint32_t* __restrict__ meta,
-                                                        double* __restrict__ grad_E) {
-    __shared__ double part[256];
-    const int64_t hi = meta[0];
-    double acc = 0.0;
-    for (int64_t j = threadIdx.x; j < hi; j += 256) acc += double(epart[j]);
-    part[threadIdx.x] = acc;
-    tree_sum_256(part);
-    if (threadIdx.x == 0) grad_E[0] += part[0];
-}
-
 // dst[c][to(i)] = src[c][from(i)]: with to_particle_order, slot i goes to particle pid[i];
 // otherwise slot i takes particle pid[i].
 __global__ __launch_bounds__(256) void k_permute_channels(const uint32_t* __restrict__ pid, const float* __restrict__ src,
@@ -1039,12 +1024,7 @@ hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
         if (e != hipSuccess) return e;
     }
     if (a.p.n > 0 && b.grad_E) {
-        // one scene: its range is [0, meta[0]), no search needed, and a small one is summed by
-        // one launch
-        if (a.p.nscenes == 1 && a.p.n <= kESingleMax) {
-            k_scene_e_single<<<1, 256, 0, a.stream>>>(b.epart, a.meta, b.grad_E);
-            return hipGetLastError();
-        }
+        // one scene: its range is [0, meta[0]), no search needed
         const int64_t* range = a.p.nscenes == 1 ? nullptr : b.erange;
         if (range) k_scene_ranges<<<unsigned((2 * a.p.nscenes + 63) / 64), 64, 0, a.stream>>>(a.keys, a.p, b.erange);
         k_scene_e_partial<<<dim3(unsigned(b.echunks), unsigned(a.p.nscenes)), 256, 0, a.stream>>>(
