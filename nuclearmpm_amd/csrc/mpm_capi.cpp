@@ -951,7 +951,7 @@ struct nclr_mpm_sim {
             *T = SegTape{base + o_states, base + o_sorts,
                          with_grid ? reinterpret_cast<float4*>(base + o_gv) : nullptr,
                          with_grid ? reinterpret_cast<float4*>(base + o_gpre) : nullptr};
-        return off;
+        return std::max<size_t>(off, 256);  // a one-step segment without grids keeps nothing
     }
 
     struct RecordInfo {
