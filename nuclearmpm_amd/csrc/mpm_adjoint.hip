@@ -859,20 +859,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // range, so a scene's dL/dE is the same alone and inside a batch.
 constexpr int kEChunk = 2048;
 
+// Boundary t of the range table (scene t / 2's first sorted position for even t, one past its last
+// for odd t): the first sorted position with keys[.] >= the boundary's key.  One wave per boundary
+// searches 64-ary: the answer lies in [lo, hi] (keys[lo - 1] < key, and hi = n or keys[hi] >= key);
+// each round the lanes probe 64 evenly spaced positions and the first probe at or above the key
+// narrows the interval to one spacing, so 2^21 keys take four rounds of one load per lane.
 __global__ __launch_bounds__(64) void k_scene_ranges(const uint32_t* __restrict__ keys, StepParams p,
                                                      int64_t* __restrict__ range) {
-    const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int t = int(blockIdx.x);
     if (t >= 2 * p.nscenes) return;
     const int sc = t >> 1;
     const uint32_t key = (t & 1) == 0 ? p.scenes[sc].tile_base
                                       : (sc + 1 < p.nscenes ? p.scenes[sc + 1].tile_base : p.ntiles);
-    int64_t lo = 0, hi = p.n;  // first sorted position with keys[.] >= key
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (keys[mid] < key) lo = mid + 1;
-        else hi = mid;
+    const int lane = int(threadIdx.x);
+    int64_t lo = 0, hi = p.n;
+    while (hi - lo > 64) {
+        const int64_t step = (hi - lo + 63) / 64;
+        // probe q; lane 63's is at or past hi - 1, and probes at or past hi load nothing
+        const int64_t q = lo + int64_t(lane + 1) * step - 1;
+        const bool at_or_above = q >= hi || keys[q] >= key;
+        const uint64_t ball = __ballot(at_or_above);
+        if (ball == 0ull) {  // every probe below the key, lane 63's at hi - 1: the answer is hi
+            lo = lo + 64 * step;
+        } else {
+            const int f = __ffsll((unsigned long long)ball) - 1;
+            const int64_t qf = lo + int64_t(f + 1) * step - 1;
+            const int64_t new_lo = f == 0 ? lo : lo + int64_t(f) * step;  // one past probe f - 1
+            hi = qf < hi ? qf : hi;
+            lo = new_lo;
+        }
     }
-    range[t] = lo;
+    // at most 64 candidates [lo, hi): the first at or above the key, else hi
+    const int64_t q = lo + lane;
+    const uint64_t ball = __ballot(q < hi && keys[q] >= key);
+    if (lane == 0) range[t] = ball == 0ull ? hi : lo + (__ffsll((unsigned long long)ball) - 1);
 }
 
 // Sum of part[0..256) by a fixed tree; the result is in part[0] for thread 0.
@@ -1026,7 +1046,7 @@ hipError_t launch_p2g_adjoint(const LaunchArgs& a, const AdjointArgs& b) {
     if (a.p.n > 0 && b.grad_E) {
         // one scene: its range is [0, meta[0]), no search needed
         const int64_t* range = a.p.nscenes == 1 ? nullptr : b.erange;
-        if (range) k_scene_ranges<<<unsigned((2 * a.p.nscenes + 63) / 64), 64, 0, a.stream>>>(a.keys, a.p, b.erange);
+        if (range) k_scene_ranges<<<unsigned(2 * a.p.nscenes), 64, 0, a.stream>>>(a.keys, a.p, b.erange);
         k_scene_e_partial<<<dim3(unsigned(b.echunks), unsigned(a.p.nscenes)), 256, 0, a.stream>>>(
             b.epart, range, a.meta, b.echunks, b.epartial);
         k_scene_e_final<<<unsigned(a.p.nscenes), 256, 0, a.stream>>>(b.epartial, range, a.meta, b.echunks, b.grad_E);

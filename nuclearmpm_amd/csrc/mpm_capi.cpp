@@ -958,11 +958,23 @@ struct nclr_mpm_sim {
         const void* base;
         int nsteps;
         bool with_grid;
-        int chunk;
-        std::vector<nclr_dev::SceneParams> scenes;
+        uint64_t params;  // params_digest() at the recording
     };
     std::vector<RecordInfo> records;  // segments recorded by advance_recorded, most recent last
     static constexpr size_t kMaxRecords = 4096;
+
+    // FNV-1a over the P2G chunk and every scene's parameters (elasticity, grid, time step), the
+    // inputs a recorded segment's steps depend on besides the state
+    uint64_t params_digest() const {
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](const void* data, size_t bytes) {
+            const unsigned char* b = static_cast<const unsigned char*>(data);
+            for (size_t k = 0; k < bytes; ++k) h = (h ^ b[k]) * 1099511628211ull;
+        };
+        mix(&p.chunk, sizeof(p.chunk));
+        mix(scenes.data(), scenes.size() * sizeof(scenes[0]));
+        return h;
+    }
 
     // Advances `nsteps` steps from the current state and records on the caller's tape `rec`
     // (record_layout bytes) what backward_segment would recompute from this state: the steps run
@@ -993,7 +1005,7 @@ struct nclr_mpm_sim {
         records.erase(std::remove_if(records.begin(), records.end(), [&](const RecordInfo& r) { return r.base == rec; }),
                       records.end());
         if (records.size() >= kMaxRecords) records.erase(records.begin());
-        records.push_back(RecordInfo{rec, nsteps, with_grid, p.chunk, scenes});
+        records.push_back(RecordInfo{rec, nsteps, with_grid, params_digest()});
         return MPM_OK;
     }
 
@@ -1005,8 +1017,7 @@ struct nclr_mpm_sim {
         auto it = std::find_if(records.begin(), records.end(), [&](const RecordInfo& r) { return r.base == rec; });
         if (it == records.end() || it->nsteps != nsteps || it->with_grid != with_grid)
             return fail(MPM_ERR_INVALID, "the tape was not recorded by mpm_advance_recorded with these nsteps and with_grid");
-        if (it->chunk != p.chunk || it->scenes.size() != scenes.size() ||
-            std::memcmp(it->scenes.data(), scenes.data(), scenes.size() * sizeof(scenes[0])) != 0)
+        if (it->params != params_digest())
             return fail(MPM_ERR_INVALID, "the engine's parameters (tuning or elasticity) changed since the tape was recorded");
         if (p.n == 0) return MPM_OK;
         SegTape T{};
