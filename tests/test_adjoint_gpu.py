@@ -298,24 +298,27 @@ def test_checkpoint_interval_does_not_change_gradients():
             np.testing.assert_array_equal(x, fresh.particles().x)
 
 
-@pytest.mark.parametrize("d", [2, 3])
-def test_recorded_segments_give_the_same_gradients(d):
+@pytest.mark.parametrize("d,material,svd", [(2, "jelly", "exact"), (3, "jelly", "exact"), (3, "snow", "accurate")])
+def test_recorded_segments_give_the_same_gradients(d, material, svd):
     """Segments recorded in the forward pass (with grids, without, or as many as fit) give the
     recomputing checkpoint schedule's final state and gradients bit for bit, leave the engine where
-    it leaves it, and a recorded tape is refused for other nsteps or after the elasticity changed."""
+    it leaves it, and a recorded tape is refused for other nsteps or after the elasticity changed.
+    Also for snow (the general G2P adjoint) in the accurate SVD mode."""
+    mat = MaterialModel.kJelly if material == "jelly" else MaterialModel.kSnow
+    ident = 0 if material == "jelly" else 1  # snow's SVD backward needs the physical identity
     if d == 2:
-        specs = [SceneSpec(Particles.from_positions(scenes.cube(2, 30, 0.3, 0.5)), MaterialModel.kJelly, 128, E=5e3)]
+        specs = [SceneSpec(Particles.from_positions(scenes.cube(2, 30, 0.3, 0.5)), mat, 128, E=5e3)]
         steps, every = 36, 7  # the last segment is one step long
     else:
-        specs = [SceneSpec(s.particles, MaterialModel.kJelly, 48, E=s.E)
+        specs = [SceneSpec(s.particles, mat, 48, E=s.E)
                  for s in scenes.jelly_3d_batched(seed=4, scenes=2, side_res=10)]
         steps, every = 20, 8
     nps = [s.particles.n for s in specs]
     targets = torch.from_numpy(np.random.default_rng(2).uniform(0.2, 0.8, (len(specs), d))).cuda()
     results = []
     for record, graph in (("none", False), ("grid", False), ("nogrid", False), ("auto", True), ("grid", True)):
-        print(f"[adjoint] recorded segments d={d} record={record} graph={graph}", flush=True)
-        with MPMBatch(specs, graph=graph) as b:
+        print(f"[adjoint] recorded segments d={d} {material} {svd} record={record} graph={graph}", flush=True)
+        with MPMBatch(specs, graph=graph, svd=svd, identity_mode=ident) as b:
             s0 = torch.cat([to_state(s.particles) for s in specs], dim=1).requires_grad_(True)
             E = torch.tensor([s.E for s in specs], dtype=torch.float64, requires_grad=True)
             out = autodiff.rollout(b, s0, E, steps, every, record=record)
@@ -331,7 +334,7 @@ def test_recorded_segments_give_the_same_gradients(d):
     np.testing.assert_array_equal(results[0][3], np.concatenate([s.particles.x for s in specs]))
     # a tape is checked against its recording
     L = lib()
-    with MPMBatch(specs) as b:
+    with MPMBatch(specs, svd=svd, identity_mode=ident) as b:
         h = b._handle()
         k = 4
         tape = torch.empty(int(L.mpm_record_bytes(h, k, 1)), dtype=torch.uint8, device="cuda")
