@@ -191,9 +191,10 @@ __global__ __launch_bounds__(256) void k_bucket_prefix(StepParams p, const uint3
 }
 
 // Bucket starts (exclusive scan of the tile totals) and the P2G work items of the new order.  On
-// entry meta[0] holds the particles on their grid in the old order (left there by the previous
-// sort).  On exit meta[0] = those of the new order (the start of the off-grid bucket), meta[1] = 0
-// (the off-grid bucket's fill counter), meta[2] = the old meta[0].  Particles off their grid take
+// entry prev_meta[0] holds the particles on their grid in the old order (left there by the
+// previous sort; prev_meta is meta itself unless the step sorts into another block).  On exit
+// meta[0] = those of the new order (the start of the off-grid bucket), meta[1] = 0 (the off-grid
+// bucket's fill counter), meta[2] = the old count.  Particles off their grid take
 // no part in P2G and G2P copies them unchanged, so their order within the last bucket cannot
 // affect any result; they are placed by an atomic counter.
 //
@@ -218,10 +219,10 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chu
                                                       uint32_t* __restrict__ tile_start,
                                                       uint32_t* __restrict__ tile_end,
                                                       const uint32_t* __restrict__ ttotal,
-                                                      uint32_t* __restrict__ meta) {
+                                                      const uint32_t* prev_meta, uint32_t* meta) {
     __shared__ uint32_t s_seen;
-    if (threadIdx.x == 0) s_seen = meta[0];
-    __syncthreads();  // meta is rewritten below
+    if (threadIdx.x == 0) s_seen = prev_meta[0];
+    __syncthreads();  // meta (possibly the same buffer as prev_meta) is rewritten below
     const BucketVisit visit{tile_start, tile_end, item_offset, p.items, chunk};
     auto count = [&](uint32_t tile) { return ttotal[tile]; };
     uint32_t cp = 0, ci = 0;
@@ -241,13 +242,14 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(StepParams p, uint32_t chu
 
 __global__ __launch_bounds__(1024) void k_bucket_round_totals(uint32_t ntiles, uint32_t chunk,
                                                               const uint32_t* __restrict__ ttotal,
-                                                              uint32_t* __restrict__ rtot, uint32_t* __restrict__ meta) {
+                                                              uint32_t* __restrict__ rtot, const uint32_t* prev_meta,
+                                                              uint32_t* meta) {
     uint32_t tp, ti;
     round_totals(blockIdx.x, ntiles, chunk, [&](uint32_t tile) { return ttotal[tile]; }, tp, ti);
     if (threadIdx.x == 0) {
         rtot[2 * blockIdx.x] = tp;
         rtot[2 * blockIdx.x + 1] = ti;
-        if (blockIdx.x == 0) meta[2] = meta[0];
+        if (blockIdx.x == 0) meta[2] = prev_meta[0];
     }
 }
 
@@ -314,17 +316,22 @@ static hipError_t bucket_dispatch(const LaunchArgs& a) {
     const auto& p = a.p;
     hipStream_t s = a.stream;
     const int64_t items = max_items(p.n, p.ntiles);
+    // the old order's work items (the previous sort's block) are read by the count and prefix
+    // kernels; the scans write the new ones (into the same block or another)
+    StepParams prev = p;
+    prev.items = const_cast<uint4*>(a.prev_items);
     k_bucket_count<D><<<unsigned(std::max<int64_t>(items, 1)), 256, 0, s>>>(
-        a.state_in, p, a.stamp, a.item_offset, a.keys_tmp, a.perm_tmp, a.uidx, a.ucnt, a.status);
+        a.state_in, prev, a.stamp, a.prev_item_offset, a.keys_tmp, a.perm_tmp, a.uidx, a.ucnt, a.status);
     if (p.ntiles > 0)
-        k_bucket_prefix<D><<<unsigned((int64_t(p.ntiles) + 3) / 4), 256, 0, s>>>(p, a.item_offset, a.ucnt, a.upref,
-                                                                                a.ttotal);
+        k_bucket_prefix<D><<<unsigned((int64_t(p.ntiles) + 3) / 4), 256, 0, s>>>(p, a.prev_item_offset, a.ucnt,
+                                                                                a.upref, a.ttotal);
     const uint32_t rounds = (p.ntiles + kScanRound - 1) / kScanRound;
     if (rounds <= 1) {
         k_bucket_scan<D><<<1, 1024, 0, s>>>(p, uint32_t(p.chunk), a.item_offset, a.tile_start, a.tile_end, a.ttotal,
-                                            a.meta);
+                                            a.prev_meta, a.meta);
     } else {
-        k_bucket_round_totals<<<rounds, 1024, 0, s>>>(p.ntiles, uint32_t(p.chunk), a.ttotal, a.rtot, a.meta);
+        k_bucket_round_totals<<<rounds, 1024, 0, s>>>(p.ntiles, uint32_t(p.chunk), a.ttotal, a.rtot, a.prev_meta,
+                                                      a.meta);
         k_bucket_rounds<D><<<rounds, 1024, 0, s>>>(p, uint32_t(p.chunk), a.item_offset, a.tile_start, a.tile_end,
                                                    a.ttotal, a.rtot, a.meta);
     }

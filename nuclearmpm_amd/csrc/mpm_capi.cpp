@@ -125,6 +125,9 @@ struct nclr_mpm_sim {
     char* sort_block = nullptr;
     size_t sort_block_bytes = 0;
     size_t so_keys = 0, so_perm = 0, so_ts = 0, so_te = 0, so_io = 0, so_items = 0, so_meta = 0;
+    // the block the next incremental sort reads the old order's work items and meta from, when it
+    // sorts into another block (nullptr: the current one)
+    char* prev_sort = nullptr;
     char* sort_tape = nullptr;  // per recomputed step of a backward segment, its sort block
     int64_t tape_states = 0;
     // per step of a backward segment, the forward grid (gv) and its pre-update sums (gpre): the
@@ -182,6 +185,9 @@ struct nclr_mpm_sim {
         a.ttotal = ttotal;
         a.meta = meta;
         a.rtot = rtot;
+        a.prev_item_offset = prev_sort ? reinterpret_cast<const uint32_t*>(prev_sort + so_io) : item_offset;
+        a.prev_items = prev_sort ? reinterpret_cast<const uint4*>(prev_sort + so_items) : p.items;
+        a.prev_meta = prev_sort ? reinterpret_cast<const uint32_t*>(prev_sort + so_meta) : meta;
         a.stream = stream;
         return a;
     }
@@ -839,21 +845,22 @@ struct nclr_mpm_sim {
     }
 
     // Steps 0 .. nsteps - 2 of a segment from the current state, each into its tape entries: step t
-    // writes state t + 1, sorts state t into its own sort block (the sort pointers are redirected;
-    // the incremental sort's inputs from the step before, the item offsets, the item table and
-    // meta, are copied in first) and, with a grid tape, writes its grid and pre-update sums there.
-    // The state pointers are left on state nsteps - 1.
+    // writes state t + 1, sorts state t into its own sort block (the sort pointers are redirected,
+    // and the incremental sort reads the old order's work items and meta from the step before's
+    // block) and, with a grid tape, writes its grid and pre-update sums there.  The state pointers
+    // are left on state nsteps - 1 and the sort pointers on its block.
     int record_steps(int nsteps, const SegTape& T, const SegGuard& g) {
         const size_t gn = gv_nodes();
+        struct PrevReset {
+            char*& prev;
+            ~PrevReset() { prev = nullptr; }
+        } reset{prev_sort};
         for (int t = 0; t + 1 < nsteps; ++t) {
             point_state(cur ^ 1, t + 1, T, g);
             gv = T.gv ? T.gv + size_t(t) * gn : g.gv;
             gpre = T.gv ? T.gpre + size_t(t) * gn : nullptr;
             char* entry = T.sorts + sort_block_bytes * size_t(t);
-            if (sorted_valid)
-                HIP_TRY(hipMemcpyAsync(entry + so_io,
-                                       (t == 0 ? sort_block : T.sorts + sort_block_bytes * size_t(t - 1)) + so_io,
-                                       sort_block_bytes - so_io, hipMemcpyDeviceToDevice, stream));
+            prev_sort = t == 0 ? sort_block : T.sorts + sort_block_bytes * size_t(t - 1);
             point_sort_block(entry);
             RC_TRY(step());
         }
@@ -995,11 +1002,12 @@ struct nclr_mpm_sim {
             pid[cur ^ 1] = g.pid[own];
             gv = g.gv;
             gpre = nullptr;
-            if (sorted_valid && nsteps > 1)
-                HIP_TRY(hipMemcpyAsync(sort_block + so_io, T.sorts + sort_block_bytes * size_t(nsteps - 2) + so_io,
-                                       sort_block_bytes - so_io, hipMemcpyDeviceToDevice, stream));
+            // the last step sorts into the engine's own block, from the step before's
+            prev_sort = nsteps > 1 ? T.sorts + sort_block_bytes * size_t(nsteps - 2) : nullptr;
             point_sort_block(sort_block);
-            RC_TRY(step());
+            const int rc = step();
+            prev_sort = nullptr;
+            RC_TRY(rc);
             g.advanced = true;
         }
         records.erase(std::remove_if(records.begin(), records.end(), [&](const RecordInfo& r) { return r.base == rec; }),

@@ -164,6 +164,11 @@ struct LaunchArgs {
     uint32_t* ttotal;          // ntiles: particles per new tile
     uint32_t* meta;            // 4 counters
     uint32_t* rtot;            // 2 per round of the tile scans: particle and work-item totals
+    // the incremental sort's inputs from the step before: its item offsets, item table and meta.
+    // The same buffers as item_offset, p.items and meta, unless the step sorts into another block.
+    const uint32_t* prev_item_offset;
+    const uint4* prev_items;
+    const uint32_t* prev_meta;
     hipStream_t stream;
 };
 
