@@ -61,28 +61,41 @@ def _ready(t: torch.Tensor) -> None:
 RECORD_MODES = ("none", "auto", "grid", "nogrid")
 
 
-def _record_plan(L, h, steps: int, every: int, record: str, device) -> list:
+def segment_sizes(steps: int, every: int) -> list:
+    """Steps of each checkpoint segment: `every` each, the last one the remainder."""
+    return [min(every, steps - d) for d in range(0, steps, every)]
+
+
+def plan_records(sizes, record: str, record_bytes, budget: float = 0.0) -> list:
     """Per segment: None (checkpoint only, recomputed in the backward pass), 0 (recorded without
-    grids) or 1 (recorded with grids).  "auto" records as many segments as fit in half the device
-    memory that is free or cached by torch, first without grids, then adds grids while they fit."""
-    sizes = [min(every, steps - d) for d in range(0, steps, every)]
+    grids) or 1 (recorded with grids).  record_bytes(k, with_grid) is a segment's tape size.
+    "auto" records as many segments as fit in `budget` bytes, first without grids, then adds
+    grids while they fit."""
     if record == "none":
         return [None] * len(sizes)
     if record in ("grid", "nogrid"):
         return [1 if record == "grid" else 0] * len(sizes)
-    free, _ = torch.cuda.mem_get_info(device)
-    budget = 0.5 * (free + torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device))
     plan, used = [None] * len(sizes), 0
     for i, k in enumerate(sizes):
-        b = int(L.mpm_record_bytes(h, k, 0))
+        b = record_bytes(k, 0)
         if used + b <= budget:
             plan[i], used = 0, used + b
     for i, k in enumerate(sizes):
         if plan[i] == 0:
-            extra = int(L.mpm_record_bytes(h, k, 1)) - int(L.mpm_record_bytes(h, k, 0))
+            extra = record_bytes(k, 1) - record_bytes(k, 0)
             if used + extra <= budget:
                 plan[i], used = 1, used + extra
     return plan
+
+
+def _record_plan(L, h, steps: int, every: int, record: str, device) -> list:
+    """plan_records for an engine; "auto" takes half the device memory that is free or cached by
+    torch as its budget."""
+    budget = 0.0
+    if record == "auto":
+        free, _ = torch.cuda.mem_get_info(device)
+        budget = 0.5 * (free + torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device))
+    return plan_records(segment_sizes(steps, every), record, lambda k, g: int(L.mpm_record_bytes(h, k, g)), budget)
 
 
 class _Rollout(torch.autograd.Function):
