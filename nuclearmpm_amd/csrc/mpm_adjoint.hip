@@ -991,23 +991,25 @@ static hipError_t g2p_adjoint_material(const LaunchArgs& a, const AdjointArgs& b
     return g2p_adjoint_one<D, PPT, TPB, true, false>(a, b);
 }
 
-// 2D: 256 threads, chunk / 256 particles each.  3D: one particle per thread (a workgroup of chunk
-// threads), at most 128 VGPRs and four waves per SIMD, where two particles per thread need 144
-// and fit three.
+// 3D: one particle per thread (a workgroup of chunk threads), at most 128 VGPRs and four waves per
+// SIMD, where two particles per thread need 144 and fit three.  2D: likewise when the particles
+// fill fewer work items than the chip has CUs (half the latency of a work item), else 256
+// threads and chunk / 256 particles each, as P2G.
 template <int D>
 static hipError_t g2p_adjoint_dispatch(const LaunchArgs& a, const AdjointArgs& b) {
-    if constexpr (D == 3) {
-        switch (a.p.chunk) {
-            case 256: return g2p_adjoint_material<D, 1, 256>(a, b);
-            case 1024: return g2p_adjoint_material<D, 1, 1024>(a, b);
-            default: return g2p_adjoint_material<D, 1, 512>(a, b);
+    if constexpr (D == 2) {
+        if (a.p.n >= int64_t(kFewItems) * a.p.chunk) {
+            switch (a.p.chunk) {
+                case 256: return g2p_adjoint_material<D, 1, 256>(a, b);
+                case 1024: return g2p_adjoint_material<D, 4, 256>(a, b);
+                default: return g2p_adjoint_material<D, 2, 256>(a, b);
+            }
         }
-    } else {
-        switch (a.p.chunk) {
-            case 256: return g2p_adjoint_material<D, 1, 256>(a, b);
-            case 1024: return g2p_adjoint_material<D, 4, 256>(a, b);
-            default: return g2p_adjoint_material<D, 2, 256>(a, b);
-        }
+    }
+    switch (a.p.chunk) {
+        case 256: return g2p_adjoint_material<D, 1, 256>(a, b);
+        case 1024: return g2p_adjoint_material<D, 1, 1024>(a, b);
+        default: return g2p_adjoint_material<D, 1, 512>(a, b);
     }
 }
 

@@ -130,6 +130,10 @@ struct CompactArgs {
 
 namespace nclr_host {
 
+// Fewer full work items than this (the chip's CUs) make a 2D launch latency-bound: P2G and the G2P
+// adjoint then run one particle per thread.
+constexpr int kFewItems = 256;
+
 struct LaunchArgs {
     int dim;
     int material;  // common material of all scenes, or nclr_dev::kMixed
