@@ -576,6 +576,8 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
         }
         float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // 2D: the forward grid kernel's four partial sums (item it of a candidate in partial it % 4)
+        float4 g1 = g, g2 = g, g3 = g;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             if (first[c] < 0) continue;
@@ -609,26 +611,57 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                 }
                 continue;
             }
-            // without the tape: both slot sets, two items' four loads in flight
-            auto add = [&](const float4 sa, const float4 sb) {
-                g.x += sa.x;
-                g.y += sa.y;
-                g.z += sa.z;
-                g.w += sa.w;
+            // without the tape: both slot sets
+            auto add = [&](float4& acc, const float4 sa, const float4 sb) {
+                acc.x += sa.x;
+                acc.y += sa.y;
+                acc.z += sa.z;
+                acc.w += sa.w;
                 b.x += sb.x;
                 b.y += sb.y;
                 b.z += sb.z;
             };
-            int it = 0;
-            for (; it + 2 <= count[c]; it += 2) {
-                const int64_t s0 = q + int64_t(it) * TL::REGION, s1 = s0 + TL::REGION;
-                const float4 a0 = slots[s0], b0 = slots_b[s0], a1 = slots[s1], b1 = slots_b[s1];
-                add(a0, b0);
-                add(a1, b1);
-            }
-            if (it < count[c]) {
+            if constexpr (D == 2) {  // four items' loads in flight, into the four partial sums
+                int it = 0;
+                for (; it + 4 <= count[c]; it += 4) {
+                    const int64_t s0 = q + int64_t(it) * TL::REGION;
+                    const float4 a0 = slots[s0], a1 = slots[s0 + TL::REGION], a2 = slots[s0 + 2 * TL::REGION],
+                                 a3 = slots[s0 + 3 * TL::REGION];
+                    const float4 b0 = slots_b[s0], b1 = slots_b[s0 + TL::REGION], b2 = slots_b[s0 + 2 * TL::REGION],
+                                 b3 = slots_b[s0 + 3 * TL::REGION];
+                    add(g, a0, b0);
+                    add(g1, a1, b1);
+                    add(g2, a2, b2);
+                    add(g3, a3, b3);
+                }
                 const int64_t s0 = q + int64_t(it) * TL::REGION;
-                add(slots[s0], slots_b[s0]);
+                if (it < count[c]) add(g, slots[s0], slots_b[s0]);
+                if (it + 1 < count[c]) add(g1, slots[s0 + TL::REGION], slots_b[s0 + TL::REGION]);
+                if (it + 2 < count[c]) add(g2, slots[s0 + 2 * TL::REGION], slots_b[s0 + 2 * TL::REGION]);
+            } else {  // items in order, two items' four loads in flight
+                int it = 0;
+                for (; it + 2 <= count[c]; it += 2) {
+                    const int64_t s0 = q + int64_t(it) * TL::REGION, s1 = s0 + TL::REGION;
+                    const float4 a0 = slots[s0], b0 = slots_b[s0], a1 = slots[s1], b1 = slots_b[s1];
+                    add(g, a0, b0);
+                    add(g, a1, b1);
+                }
+                if (it < count[c]) {
+                    const int64_t s0 = q + int64_t(it) * TL::REGION;
+                    add(g, slots[s0], slots_b[s0]);
+                }
+            }
+        }
+        if constexpr (D == 2) {
+            if (!gpre) {  // the partial sums added in order, as the forward grid kernel adds them
+                const float4 parts[3] = {g1, g2, g3};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    g.x += parts[k].x;
+                    g.y += parts[k].y;
+                    g.z += parts[k].z;
+                    g.w += parts[k].w;
+                }
             }
         }
         if (gpre) g = gpre[int64_t(nt_idx) * TL::NODES + l];  // the same sums, in the same order

@@ -294,10 +294,14 @@ __global__ __launch_bounds__(TPB) void k_p2g_tile(const float* __restrict__ st, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// grid update (src/nclr.h:263-310): one workgroup per node tile
+// grid update (src/nclr.h:263-310): one workgroup per node tile (256 threads; 1024 for a 2D launch
+// with few work items).  In 3D a node sums its candidate tiles' work items in order; in 2D in four
+// partial sums, partial w over items w, w + 4, ... of every candidate tile in order, added in
+// partial order: four independent chains of additions behind the slot loads (the 256-scene 2D
+// batch 3.6 % faster; in 3D it measured slower).  The grid adjoint re-sums in the same order.
 // ---------------------------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ slots,
+__global__ __launch_bounds__(D == 2 ? 1024 : 256) void k_grid_tiles(const float4* __restrict__ slots,
                                                     const uint32_t* __restrict__ item_offset, StepParams p,
                                                     float4* __restrict__ gv, float4* __restrict__ gpre,
                                                     uint32_t* __restrict__ stamps, uint32_t* __restrict__ active) {
@@ -344,6 +348,65 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
         count[c] = s_count[c];
     }
     uint32_t massive_nodes = 0;
+    auto add = [](float4& acc, const float4 v) {
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+    };
+    if constexpr (D == 2) {
+        // With 1024 threads (a launch with few work items) quarter w of the workgroup computes
+        // partial sum w, which quarters the chain of slot loads a node waits on; with 256 threads
+        // each thread computes all four.  Both give the same sums.
+        static_assert(TL::NODES == 256, "one node per thread of a 2D node tile");
+        __shared__ float4 s_part[3][TL::NODES];
+        const int l = int(threadIdx.x) & (TL::NODES - 1);
+        const int n[2] = {nto[0] * TL::S + (l >> TL::LOG), nto[1] * TL::S + (l & (TL::S - 1))};
+        const bool split = blockDim.x == 4 * TL::NODES;
+        const int w = split ? int(threadIdx.x) >> 8 : 0;
+        float4 g0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g1 = g0, g2 = g0, g3 = g0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (first[c] < 0) continue;
+            const int r0 = n[0] - (nto[0] - (c & 1)) * TL::S, r1 = n[1] - (nto[1] - ((c >> 1) & 1)) * TL::S;
+            if (r0 >= TL::R || r1 >= TL::R) continue;
+            const float4* q = slots + int64_t(first[c]) * TL::REGION + (r0 * TL::R + r1);
+            if (split) {
+                for (int it = w; it < count[c]; it += 4) add(g0, q[int64_t(it) * TL::REGION]);
+            } else {
+                int it = 0;
+                for (; it + 4 <= count[c]; it += 4) {
+                    const float4 s0 = q[int64_t(it) * TL::REGION], s1 = q[int64_t(it + 1) * TL::REGION],
+                                 s2 = q[int64_t(it + 2) * TL::REGION], s3 = q[int64_t(it + 3) * TL::REGION];
+                    add(g0, s0);
+                    add(g1, s1);
+                    add(g2, s2);
+                    add(g3, s3);
+                }
+                if (it < count[c]) add(g0, q[int64_t(it) * TL::REGION]);
+                if (it + 1 < count[c]) add(g1, q[int64_t(it + 1) * TL::REGION]);
+                if (it + 2 < count[c]) add(g2, q[int64_t(it + 2) * TL::REGION]);
+            }
+        }
+        if (split) {
+            if (w > 0) s_part[w - 1][l] = g0;
+            __syncthreads();
+            if (w == 0) {
+                g1 = s_part[0][l];
+                g2 = s_part[1][l];
+                g3 = s_part[2][l];
+            }
+        }
+        if (w == 0) {
+            float4 g = g0;
+            add(g, g1);
+            add(g, g2);
+            add(g, g3);
+            massive_nodes += g.w > 0.0f ? 1u : 0u;
+            if (gpre) gpre[int64_t(nt_idx) * TL::NODES + l] = g;
+            gv[int64_t(nt_idx) * TL::NODES + l] = grid_node<D>(g, n, sp);
+        }
+    } else
     for (int l = threadIdx.x; l < TL::NODES; l += blockDim.x) {
         int n[D];
         {
@@ -372,30 +435,12 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const float4* __restrict__ s
             int it = 0;
             for (; it + 4 <= count[c]; it += 4, q += 4 * TL::REGION) {
                 const float4 s0 = q[0], s1 = q[TL::REGION], s2 = q[2 * TL::REGION], s3 = q[3 * TL::REGION];
-                g.x += s0.x;
-                g.y += s0.y;
-                g.z += s0.z;
-                g.w += s0.w;
-                g.x += s1.x;
-                g.y += s1.y;
-                g.z += s1.z;
-                g.w += s1.w;
-                g.x += s2.x;
-                g.y += s2.y;
-                g.z += s2.z;
-                g.w += s2.w;
-                g.x += s3.x;
-                g.y += s3.y;
-                g.z += s3.z;
-                g.w += s3.w;
+                add(g, s0);
+                add(g, s1);
+                add(g, s2);
+                add(g, s3);
             }
-            for (; it < count[c]; ++it, q += TL::REGION) {
-                const float4 s = q[0];
-                g.x += s.x;
-                g.y += s.y;
-                g.z += s.z;
-                g.w += s.w;
-            }
+            for (; it < count[c]; ++it, q += TL::REGION) add(g, q[0]);
         }
         massive_nodes += g.w > 0.0f ? 1u : 0u;
         if (gpre) gpre[int64_t(nt_idx) * TL::NODES + l] = g;
@@ -681,9 +726,9 @@ hipError_t launch_g2p(const LaunchArgs& a) { return a.dim == 2 ? g2p_dispatch<2>
 hipError_t launch_grid(const LaunchArgs& a) {
     const unsigned tiles = a.p.node_tiles;
     if (tiles == 0) return hipSuccess;
-    if (a.dim == 2)
-        k_grid_tiles<2><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.gpre, a.stamps,
-                                                                a.active);
+    if (a.dim == 2)  // few work items: four threads per node (the same sums)
+        k_grid_tiles<2><<<xcd_grid(tiles), a.p.n < int64_t(kFewItems) * a.p.chunk ? 1024 : 256, 0, a.stream>>>(
+            a.slots, a.item_offset, a.p, a.gv, a.gpre, a.stamps, a.active);
     else
         k_grid_tiles<3><<<xcd_grid(tiles), 256, 0, a.stream>>>(a.slots, a.item_offset, a.p, a.gv, a.gpre, a.stamps,
                                                                 a.active);
