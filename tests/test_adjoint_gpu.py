@@ -377,6 +377,34 @@ def test_batched_gradients_equal_single_scene_gradients():
         start += nps[s]
 
 
+def test_large_2d_batch_gradients_equal_single_scene_gradients():
+    """A 2D batch past 256 full work items takes the two-particles-per-thread P2G and G2P adjoint
+    and the 256-thread grid kernel; each scene alone (a small launch) takes one particle per thread
+    and the 1024-thread grid kernel.  Both give the same gradients bit for bit."""
+    base = affine_jelly(2, 150, 0.3, 0.6, seed=21)  # 22,500 particles
+    specs = [SceneSpec(base, MaterialModel.kJelly, 128, E=e) for e in (3e3, 4e3, 5e3, 6e3, 7e3, 8e3)]
+    nps = [s.particles.n for s in specs]
+    assert sum(nps) >= 256 * 512
+    targets = torch.from_numpy(np.random.default_rng(5).uniform(0.2, 0.8, (len(specs), 2))).cuda()
+    steps, every = 6, 3
+    print(f"[adjoint] large 2D batch of {len(specs)} x {nps[0]} particles", flush=True)
+    with MPMBatch(specs, p2g_chunk=512) as b:
+        s0 = torch.cat([to_state(s.particles) for s in specs], dim=1).requires_grad_(True)
+        E = torch.tensor([s.E for s in specs], dtype=torch.float64, requires_grad=True)
+        com_loss(autodiff.rollout(b, s0, E, steps, every), 2, targets, nps).backward()
+        g_batch = s0.grad.cpu().numpy()
+        gE_batch = E.grad.numpy()
+    start = 0
+    for s, sp in enumerate(specs):
+        with MPMSimulation(sp.particles, MaterialModel.kJelly, res=128, E=sp.E, p2g_chunk=512) as sim:
+            s1 = to_state(sp.particles).requires_grad_(True)
+            E1 = torch.tensor([sp.E], dtype=torch.float64, requires_grad=True)
+            com_loss(autodiff.rollout(sim, s1, E1, steps, every), 2, targets[s:s + 1], [sp.particles.n]).backward()
+            np.testing.assert_array_equal(g_batch[:, start:start + nps[s]], s1.grad.cpu().numpy())
+            assert gE_batch[s] == float(E1.grad), (s, gE_batch[s], float(E1.grad))
+        start += nps[s]
+
+
 # ---- snow and liquid -----------------------------------------------------------------------
 
 def plastic_particles(d, side, seed):
