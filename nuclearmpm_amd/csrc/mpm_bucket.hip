@@ -173,8 +173,16 @@ __global__ __launch_bounds__(256) void k_bucket_prefix(StepParams p, const uint3
     }
     const int o = B::NEIGH - 1 - s;  // the move from source tile to k
     const uint32_t u0 = inside ? item_offset[t] : 0u, u1 = inside ? item_offset[t + 1] : 0u;
+    // the source items' counts, four loads in flight (integer sums: the order is free)
+    const uint32_t* cnt = ucnt + o;
     uint32_t mine = 0;
-    for (uint32_t u = u0; u < u1; ++u) mine += ucnt[int64_t(u) * B::NO + o];
+    uint32_t u = u0;
+    for (; u + 4 <= u1; u += 4) {
+        const uint32_t c0 = cnt[int64_t(u) * B::NO], c1 = cnt[int64_t(u + 1) * B::NO],
+                       c2 = cnt[int64_t(u + 2) * B::NO], c3 = cnt[int64_t(u + 3) * B::NO];
+        mine += (c0 + c1) + (c2 + c3);
+    }
+    for (; u < u1; ++u) mine += cnt[int64_t(u) * B::NO];
     uint32_t x = mine;
 #pragma unroll
     for (int off = 1; off < 32; off <<= 1) {  // 3^D <= 27 lanes take part
@@ -182,9 +190,19 @@ __global__ __launch_bounds__(256) void k_bucket_prefix(StepParams p, const uint3
         if (s >= off) x += y;
     }
     uint32_t run = x - mine;
-    for (uint32_t u = u0; u < u1; ++u) {
+    u = u0;
+    for (; u + 4 <= u1; u += 4) {
+        const uint32_t c0 = cnt[int64_t(u) * B::NO], c1 = cnt[int64_t(u + 1) * B::NO],
+                       c2 = cnt[int64_t(u + 2) * B::NO], c3 = cnt[int64_t(u + 3) * B::NO];
         upref[int64_t(u) * B::NO + o] = run;
-        run += ucnt[int64_t(u) * B::NO + o];
+        upref[int64_t(u + 1) * B::NO + o] = run + c0;
+        upref[int64_t(u + 2) * B::NO + o] = run + c0 + c1;
+        upref[int64_t(u + 3) * B::NO + o] = run + c0 + c1 + c2;
+        run += (c0 + c1) + (c2 + c3);
+    }
+    for (; u < u1; ++u) {
+        upref[int64_t(u) * B::NO + o] = run;
+        run += cnt[int64_t(u) * B::NO];
     }
     const uint32_t total = __shfl(x, B::NEIGH - 1);
     if (s == 0) ttotal[k] = total;
