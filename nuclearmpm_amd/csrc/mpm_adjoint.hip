@@ -960,8 +960,17 @@ __global__ __launch_bounds__(256) void k_scene_e_partial(const float* __restrict
     scene_range(range, meta, sc, s_lo, s_hi);
     const int64_t lo = s_lo + int64_t(blockIdx.x) * kEChunk;
     const int64_t hi = min(s_hi, lo + int64_t(kEChunk));
+    // the thread's elements j, j + 256, ... summed in that order, eight loads in flight
     double acc = 0.0;
-    for (int64_t j = lo + threadIdx.x; j < hi; j += 256) acc += double(epart[j]);
+    int64_t j = lo + threadIdx.x;
+    for (; j + 7 * 256 < hi; j += 8 * 256) {
+        float e[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) e[u] = epart[j + u * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += double(e[u]);
+    }
+    for (; j < hi; j += 256) acc += double(epart[j]);
     part[threadIdx.x] = acc;
     tree_sum_256(part);
     if (threadIdx.x == 0) partial[int64_t(sc) * chunks + blockIdx.x] = part[0];
